@@ -1,0 +1,150 @@
+#!/usr/bin/env python3
+"""Benchmark: calcium_sparks time stepping on one MI355X.
+
+Workload (the single-instance configuration): a 1200x1200x600 nm dyad at
+3 nm (400x400x200 = 32M voxels), 5 species slots (7 species), cleft 12 nm,
+jSR 30 nm over 600x600 nm, 100 RyRs by seeded Poisson-disk sampling in a
+400 nm disk (30% pRyR), stochastic Philox gating, FP32 fields.  A step is
+one full time step of the scheme (gating, stencil + reaction, boundary
+fluxes).  Inputs are synthetic and resident in HBM before timing starts.
+
+Prints one JSON line.  `value` is the whole-step algorithmic HBM
+bandwidth: 41 B per voxel-step (1 B domain id + 5 species x (4 B read +
+4 B write)) x voxels / step time.  `roofline` prices the fused
+stencil/reaction kernel alone, timed with HIP events on its own stream.
+`cpu_baseline` runs the FP64 C oracle on the host cores for a bounded
+sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "hpc-ca-phospattern_amd"))
+
+import numpy as np  # noqa: E402
+
+import csparks as cs  # noqa: E402
+import dyad  # noqa: E402
+
+PEAK_HBM_GBS = 8000.0
+BYTES_F32 = 1 + 5 * (4 + 4)
+BYTES_F64 = 1 + 5 * (8 + 8)
+METRIC = "spark realisations/sec (closure vs plain) at KS<0.05; stencil HBM GB/s %peak"
+TSTOP_MS = 30.0
+PROFILE_TRAFFIC = os.path.join(REPO, "profiles", "traffic.json")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(d, budget_s=15.0):
+    """FP64 oracle on the host cores, a bounded number of steps of the same
+    grid; returns voxel-steps/s."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle_py as op
+    opts = cs.Options(h=3.0, seed=1, open={"ryr": [0]})
+    orc = op.Oracle(d.geometry, d.model, opts)
+    nv = int(np.prod(orc.n))
+    orc.step(1)  # first touch of the fields
+    t0 = time.perf_counter()
+    steps = 0
+    while steps < 3 or (time.perf_counter() - t0 < budget_s and steps < 200):
+        orc.step(1)
+        steps += 1
+    el = time.perf_counter() - t0
+    orc.close()
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return nv * steps / el, cores, steps, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--size", type=int, nargs=3, default=[400, 400, 200])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "
+              "(independent realisations would shard by instance, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    t0 = time.perf_counter()
+    d = dyad.config1(seed=1, n=tuple(args.size))
+    opts = cs.Options(h=3.0, seed=1, open={"ryr": [0]}, precision=args.precision, tstop=TSTOP_MS)
+    sim = cs.Simulation(d.geometry, d.model, opts)
+    log(f"setup {time.perf_counter() - t0:.1f}s; {cs.device_info()}; grid {sim.n}")
+    nv = int(np.prod(sim.n))
+    bpv = BYTES_F32 if args.precision == "f32" else BYTES_F64
+
+    sim.step(args.warmup)
+    sim.synchronize()
+    t1 = time.perf_counter()
+    total_ms, kernel_ms = sim.time_steps(args.steps)
+    sim.synchronize()
+    wall = time.perf_counter() - t1
+    ms_per_step = wall * 1e3 / args.steps
+    gbs = nv * bpv / (ms_per_step * 1e-3) / 1e9
+    kern_gbs = nv * bpv / (kernel_ms * 1e-3) / 1e9
+    steps_per_real = cs.host_lib().cs_count_steps(TSTOP_MS, sim.dt)
+    log(f"{args.steps} steps: wall {wall * 1e3:.1f} ms, events {total_ms:.1f} ms, "
+        f"stencil kernel {kernel_ms * 1e3:.1f} us ({kern_gbs:.0f} GB/s)")
+
+    # conservation check of the timed run (total calcium, every slot)
+    tot = sim.stats()[2].sum()
+    sim.close()
+
+    traffic = None
+    if os.path.exists(PROFILE_TRAFFIC):
+        with open(PROFILE_TRAFFIC) as f:
+            t = json.load(f)
+        if t.get("workload") == "config1" and t.get("precision") == args.precision and tuple(t.get("grid", ())) == tuple(args.size):
+            traffic = t.get("bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        vps, cores, csteps, cel = cpu_baseline(d)
+        cpu = {"value": round(vps * BYTES_F32 / 1e9, 3), "unit": "GB/s", "cores": cores, "kind": "port",
+               "sample": f"FP64 C oracle, {csteps} steps of the same {args.size[0]}x{args.size[1]}x{args.size[2]} grid "
+                         f"in {cel:.1f} s ({vps:.3e} voxel-steps/s, priced at {BYTES_F32} B/voxel-step)"}
+
+    out = {
+        "metric": METRIC,
+        "value": round(gbs, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == "f32" else "f64",
+        "data": "synthetic",
+        "config": {"workload": "config1: single 400x400x200 dyad @3nm, 7 species/5 slots, 100 RyR "
+                               "(30% pRyR), stochastic gating, FP32 fields",
+                   "grid": list(args.size), "voxels": nv,
+                   "bytes_per_voxel_step": bpv},
+        "frac_of_peak": round(gbs / PEAK_HBM_GBS, 4),
+        "voxel_steps_per_s": round(nv / (ms_per_step * 1e-3), 1),
+        "spark_realisations_per_s": 1.0 / (steps_per_real * ms_per_step * 1e-3),
+        "roofline": {"bound": "hbm", "achieved": round(kern_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(kern_gbs / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "kernel": "k_stencil", "kernel_ms": round(kernel_ms, 5),
+                     "algorithmic_bytes_per_launch": nv * bpv},
+        "cpu_baseline": cpu,
+        "total_calcium": tot,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

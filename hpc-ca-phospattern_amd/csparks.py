@@ -1,0 +1,642 @@
+"""Python mirror of the calcium_sparks MI355X path (ctypes over the C ABI).
+
+The reference program is C; this module exists so tests and the benchmark
+can drive the same C ABI a C caller uses (include/csparks.h, cs_host.h):
+
+* ``Geometry`` / ``SpeciesModel`` / ``Options`` hold what the reference
+  reads from its geometry file, species file and command line
+  (geometry.c:92-141, species.c:103-258, options.c:502-542);
+* ``Simulation`` builds the problem with libcshost and steps it on the GPU
+  with libcsparks: ``step`` is the reference loop body
+  (calcium_sparks.c:196-213), ``species`` reads a field back in the
+  reference layout, ``stats`` gives the per-domain min/max/sum of
+  Species_output_scalar_data (species.c:2785-2863).
+
+There is no CPU fallback: constructing a Simulation without the built HIP
+library, or without a GPU, raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD = os.path.join(HERE, "_build")
+
+MAX_DOMAINS = 5
+MAX_SPECIES = 8
+MAX_BOUNDARIES = 8
+MAX_NAMED = 8
+MAX_OPEN = 64
+NAME_LEN = 32
+MAX_FLUXES = 4
+
+PREC_F64, PREC_F32 = 0, 1
+RNG_PHILOX, RNG_MT = 0, 1
+BOUNDARY_DENSITY, BOUNDARY_DISCRETE = 0, 1
+FLUX_RYR, FLUX_SERCA = 0, 1
+
+Name = C.c_char * NAME_LEN
+
+
+class CGeometry(C.Structure):
+    _fields_ = [
+        ("N", C.c_uint64 * 3),
+        ("h", C.c_double),
+        ("global_size", C.c_double * 3),
+        ("num_domains", C.c_uint32),
+        ("domain_names", Name * MAX_DOMAINS),
+        ("domain_indices", C.c_uint8 * MAX_DOMAINS),
+        ("domain_connections", C.c_uint8 * (MAX_DOMAINS * MAX_DOMAINS)),
+        ("voxels", C.POINTER(C.c_uint8)),
+        ("num_boundaries", C.c_uint32),
+        ("boundary_names", Name * MAX_BOUNDARIES),
+        ("boundary_types", C.c_uint32 * MAX_BOUNDARIES),
+        ("boundary_sizes", C.c_uint64 * MAX_BOUNDARIES),
+        ("boundaries", C.POINTER(C.c_uint16) * MAX_BOUNDARIES),
+    ]
+
+
+class CChannelModel(C.Structure):
+    _fields_ = [
+        ("Kd_open", C.c_double), ("k_min_open", C.c_double), ("k_max_open", C.c_double),
+        ("n_open", C.c_double), ("Kd_close", C.c_double), ("k_min_close", C.c_double),
+        ("k_max_close", C.c_double), ("n_close", C.c_double), ("i_unitary", C.c_double),
+        ("boundary", Name), ("species", Name),
+    ]
+
+
+class CSercaModel(C.Structure):
+    _fields_ = [
+        ("v_cyt_to_A_sr", C.c_double), ("density", C.c_double), ("scale", C.c_double),
+        ("boundary", Name), ("species", Name),
+    ]
+
+
+class CSpeciesModel(C.Structure):
+    _fields_ = [
+        ("num_domains", C.c_uint32), ("num_species", C.c_uint32),
+        ("domain_names", Name * MAX_DOMAINS),
+        ("species_names", Name * MAX_SPECIES),
+        ("num_diffusive", C.c_uint32 * MAX_DOMAINS),
+        ("diffusive", (C.c_uint8 * MAX_SPECIES) * MAX_DOMAINS),
+        ("sigma", (C.c_double * MAX_SPECIES) * MAX_DOMAINS),
+        ("init", (C.c_double * MAX_SPECIES) * MAX_DOMAINS),
+        ("num_buffers", C.c_uint32 * MAX_DOMAINS),
+        ("k_on", (C.c_double * MAX_SPECIES) * MAX_DOMAINS),
+        ("k_off", (C.c_double * MAX_SPECIES) * MAX_DOMAINS),
+        ("tot", (C.c_double * MAX_SPECIES) * MAX_DOMAINS),
+        ("buffer_species", ((C.c_uint8 * 2) * MAX_SPECIES) * MAX_DOMAINS),
+        ("use_ryr", C.c_uint32), ("use_p_ryr", C.c_uint32), ("use_serca", C.c_uint32),
+        ("ryr", CChannelModel), ("p_ryr", CChannelModel), ("serca", CSercaModel),
+        ("conv", C.c_double * 3),
+        ("num_fixed", C.c_uint32),
+        ("fixed", C.c_uint8 * (2 * MAX_DOMAINS * MAX_SPECIES)),
+    ]
+
+
+class COpenEntry(C.Structure):
+    _fields_ = [("name", Name), ("count", C.c_uint32), ("idx", C.c_int32 * MAX_OPEN)]
+
+
+class CCloseEntry(C.Structure):
+    _fields_ = [("name", Name), ("time", C.c_double)]
+
+
+class COptions(C.Structure):
+    _fields_ = [
+        ("tstop", C.c_double), ("h", C.c_double), ("dt", C.c_double), ("dt_save", C.c_double),
+        ("dt_update_react", C.c_double), ("dt_update_stoch", C.c_double), ("tabort", C.c_double),
+        ("force_dirichlet", C.c_int32), ("seed_given", C.c_int32), ("seed", C.c_uint32),
+        ("num_open", C.c_uint32), ("open", COpenEntry * MAX_NAMED),
+        ("num_t_close", C.c_uint32), ("t_close", CCloseEntry * MAX_NAMED),
+        ("precision", C.c_uint32), ("rng", C.c_uint32),
+    ]
+
+
+class CFluxDesc(C.Structure):
+    _fields_ = [
+        ("kind", C.c_uint32), ("species", C.c_uint32), ("substep", C.c_uint32),
+        ("inner", C.c_uint32), ("discrete", C.c_uint32), ("num_pairs", C.c_uint32),
+        ("pairs", C.POINTER(C.c_uint32)), ("params", C.c_double * 10),
+        ("init_states", C.POINTER(C.c_uint8)),
+    ]
+
+
+class CProblem(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint32 * 3), ("num_domains", C.c_uint32), ("num_species", C.c_uint32),
+        ("precision", C.c_uint32), ("rng", C.c_uint32), ("seed", C.c_uint64),
+        ("instance", C.c_uint64), ("h", C.c_double), ("dt", C.c_double), ("dV", C.c_double),
+        ("domain_ids", C.POINTER(C.c_uint8)),
+        ("init", (C.c_double * MAX_SPECIES) * MAX_DOMAINS),
+        ("substeps", C.c_uint32 * MAX_SPECIES),
+        ("coef", (C.c_double * (MAX_DOMAINS * MAX_DOMAINS)) * MAX_SPECIES),
+        ("num_buffers", C.c_uint32), ("reaction_substep", C.c_uint32), ("dt_react", C.c_double),
+        ("bsp0", (C.c_uint8 * MAX_DOMAINS) * MAX_SPECIES),
+        ("bsp1", (C.c_uint8 * MAX_DOMAINS) * MAX_SPECIES),
+        ("k_on", (C.c_double * MAX_DOMAINS) * MAX_SPECIES),
+        ("k_off", (C.c_double * MAX_DOMAINS) * MAX_SPECIES),
+        ("tot", (C.c_double * MAX_DOMAINS) * MAX_SPECIES),
+        ("num_fluxes", C.c_uint32), ("fluxes", CFluxDesc * MAX_FLUXES),
+        ("stochastic_substep", C.c_uint32), ("force_dirichlet", C.c_uint32),
+        ("fixed", (C.c_uint8 * MAX_SPECIES) * MAX_DOMAINS),
+    ]
+
+
+class CMT(C.Structure):
+    _fields_ = [("mt", C.c_uint32 * 624), ("idx", C.c_uint32)]
+
+
+class CDiscreteInfo(C.Structure):
+    _fields_ = [("name", Name), ("boundary", C.c_uint32), ("flux", C.c_int32), ("size", C.c_uint32)]
+
+
+class CSetup(C.Structure):
+    _fields_ = [
+        ("problem", CProblem), ("subdivisions", C.c_uint32), ("h_geom", C.c_double),
+        ("N", C.c_uint64 * 3), ("num_domains", C.c_uint32), ("num_species", C.c_uint32),
+        ("domain_names", Name * MAX_DOMAINS), ("species_names", Name * MAX_SPECIES),
+        ("volumes", C.c_double * MAX_DOMAINS), ("domain_voxels", C.c_int64 * MAX_DOMAINS),
+        ("num_discrete", C.c_uint32), ("discrete", CDiscreteInfo * MAX_BOUNDARIES),
+        ("num_diffusive_species", C.c_uint32), ("all_diffusive", C.c_uint8 * MAX_SPECIES),
+        ("has_ryr_boundary", C.c_uint32), ("has_p_ryr_boundary", C.c_uint32),
+        ("ryr_discrete", C.c_int32), ("p_ryr_discrete", C.c_int32),
+        ("mt", CMT), ("seed", C.c_uint32),
+    ]
+
+
+class CAbortState(C.Structure):
+    _fields_ = [("tabort", C.c_double), ("last_closed", C.c_double),
+                ("all_closed", C.c_int), ("active", C.c_int)]
+
+
+# --------------------------------------------------------------- libraries
+_libs: Dict[str, C.CDLL] = {}
+
+
+def _load(name: str) -> C.CDLL:
+    if name in _libs:
+        return _libs[name]
+    path = os.path.join(BUILD, name)
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built; run hpc-ca-phospattern_amd/build.py")
+    lib = C.CDLL(path)
+    _libs[name] = lib
+    return lib
+
+
+def host_lib() -> C.CDLL:
+    lib = _load("libcshost.so")
+    if not getattr(lib, "_cs_typed", False):
+        lib.cs_setup_build.restype = C.c_int
+        lib.cs_setup_build.argtypes = [C.POINTER(CGeometry), C.POINTER(CSpeciesModel),
+                                       C.POINTER(COptions), C.POINTER(C.POINTER(CSetup)),
+                                       C.c_char_p, C.c_size_t]
+        lib.cs_setup_free.argtypes = [C.POINTER(CSetup)]
+        lib.cs_setup_free.restype = None
+        lib.cs_setup_mt_uniforms.argtypes = [C.POINTER(CSetup), C.c_uint64, C.c_uint64,
+                                             C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
+        lib.cs_options_defaults.argtypes = [C.POINTER(COptions)]
+        lib.cs_options_defaults.restype = None
+        lib.cs_mt_seed.argtypes = [C.POINTER(CMT), C.c_uint32]
+        lib.cs_mt_seed.restype = None
+        lib.cs_mt_u32.argtypes = [C.POINTER(CMT)]
+        lib.cs_mt_u32.restype = C.c_uint32
+        lib.cs_mt_ldrand.argtypes = [C.POINTER(CMT)]
+        lib.cs_mt_ldrand.restype = C.c_double
+        lib.cs_save_interval.argtypes = [C.c_double, C.c_double]
+        lib.cs_save_interval.restype = C.c_uint64
+        lib.cs_count_steps.argtypes = [C.c_double, C.c_double]
+        lib.cs_count_steps.restype = C.c_uint64
+        lib.cs_abort_init.argtypes = [C.POINTER(CAbortState), C.c_double, C.c_int]
+        lib.cs_abort_init.restype = None
+        lib.cs_abort_snapshot.argtypes = [C.POINTER(CAbortState), C.c_int]
+        lib.cs_abort_snapshot.restype = None
+        lib.cs_abort_check.argtypes = [C.POINTER(CAbortState), C.c_double]
+        lib.cs_abort_check.restype = C.c_int
+        lib.cs_host_sizeof.argtypes = [C.c_int]
+        lib.cs_host_sizeof.restype = C.c_size_t
+        lib._cs_typed = True
+    return lib
+
+
+def device_lib() -> C.CDLL:
+    lib = _load("libcsparks.so")
+    if not getattr(lib, "_cs_typed", False):
+        P = C.c_void_p
+        lib.cs_create.argtypes = [C.POINTER(CProblem), C.POINTER(P)]
+        lib.cs_destroy.argtypes = [P]
+        lib.cs_reset.argtypes = [P]
+        lib.cs_upload_species.argtypes = [P, C.c_uint32, C.POINTER(C.c_double)]
+        lib.cs_download_species.argtypes = [P, C.c_uint32, C.POINTER(C.c_double)]
+        lib.cs_step.argtypes = [P, C.c_uint64, C.c_uint64]
+        lib.cs_step_uniforms.argtypes = [P, C.c_uint64, C.c_uint64, C.POINTER(C.c_double), C.c_uint64]
+        lib.cs_uniforms_needed.argtypes = [P, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64)]
+        lib.cs_synchronize.argtypes = [P]
+        lib.cs_reduce_stats.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        lib.cs_get_channel_states.argtypes = [P, C.c_uint32, C.POINTER(C.c_uint8)]
+        lib.cs_set_channel_states.argtypes = [P, C.c_uint32, C.POINTER(C.c_uint8)]
+        lib.cs_time_steps.argtypes = [P, C.c_uint64, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        lib.cs_device_info.argtypes = [C.POINTER(C.c_int), C.c_char_p, C.c_size_t]
+        lib.cs_ryr_flux.argtypes = [C.c_double] * 4 + [C.POINTER(C.c_double)]
+        lib.cs_ryr_flux.restype = C.c_double
+        lib.cs_serca_flux.argtypes = [C.c_double] * 4 + [C.POINTER(C.c_double)]
+        lib.cs_serca_flux.restype = C.c_double
+        lib.cs_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        lib.cs_philox4x32_10.restype = None
+        lib.cs_last_error.restype = C.c_char_p
+        lib._cs_typed = True
+    return lib
+
+
+class CSError(RuntimeError):
+    pass
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        msg = device_lib().cs_last_error().decode(errors="replace")
+        raise CSError(f"{what} failed ({rc}): {msg}")
+
+
+# ------------------------------------------------------------ input model
+@dataclass
+class Boundary:
+    name: str
+    type: str                      # "density" | "discrete"
+    pairs: np.ndarray              # uint16 [n, 6] geometry voxels (from, to)
+
+
+@dataclass
+class Geometry:
+    """Content of a reference geometry file (geometry.c:92-141, :487-1011)."""
+    voxels: np.ndarray             # uint8 [N0, N1, N2], z fastest
+    h: float
+    domain_names: List[str]
+    domain_connections: np.ndarray  # uint8 [nd, nd]
+    boundaries: List[Boundary] = field(default_factory=list)
+    global_size: Optional[Tuple[float, float, float]] = None
+
+    def boundary(self, name: str) -> Boundary:
+        for b in self.boundaries:
+            if b.name == name:
+                return b
+        raise KeyError(name)
+
+
+@dataclass
+class ChannelModel:
+    Kd_open: float
+    k_min_open: float
+    k_max_open: float
+    n_open: float
+    Kd_close: float
+    k_min_close: float
+    k_max_close: float
+    n_close: float
+    i_unitary: float
+    boundary: str
+    species: str = "Ca"
+
+
+@dataclass
+class SercaModel:
+    v_cyt_to_A_sr: float
+    density: float
+    scale: float
+    boundary: str
+    species: str = "Ca"
+
+
+@dataclass
+class Buffer:
+    bound: int                     # species index of the buffer
+    free: int                      # species index it binds (Ca)
+    k_on: float
+    k_off: float
+    tot: float
+
+
+@dataclass
+class SpeciesModel:
+    """Content of a reference species file (species.c:103-258)."""
+    domain_names: List[str]
+    species_names: List[str]
+    diffusive: List[List[int]]     # per domain: species indices
+    sigma: List[List[float]]       # per domain: nm^2/ms, same order
+    init: np.ndarray               # [nd, ns] uM
+    buffers: List[List[Buffer]]    # per domain
+    ryr: Optional[ChannelModel] = None
+    p_ryr: Optional[ChannelModel] = None
+    serca: Optional[SercaModel] = None
+    conv: Tuple[float, float, float] = (28853.9, 28853.9, 115416.0)
+    fixed: List[Tuple[int, int]] = field(default_factory=list)
+
+
+@dataclass
+class Options:
+    """Command-line options of the reference (options.c:502-542)."""
+    tstop: float = 10.0
+    h: float = 6.0
+    dt: float = -1.0
+    dt_save: float = 0.5
+    dt_update_react: float = -1.0
+    dt_update_stoch: float = -1.0
+    tabort: float = 0.0
+    force_dirichlet: bool = False
+    seed: Optional[int] = None
+    open: Dict[str, Sequence[int]] = field(default_factory=dict)
+    t_close: Dict[str, float] = field(default_factory=dict)
+    precision: str = "f64"         # "f64" | "f32"
+    rng: str = "philox"            # "philox" | "mt"
+
+
+def _setname(dst, s: str):
+    b = s.encode()
+    if len(b) >= NAME_LEN:
+        raise ValueError(f"name too long: {s}")
+    dst.value = b
+
+
+def _setfield(struct, fname: str, s: str):
+    b = s.encode()
+    if len(b) >= NAME_LEN:
+        raise ValueError(f"name too long: {s}")
+    setattr(struct, fname, b)
+
+
+class _Holder:
+    """Keeps numpy buffers alive while C structs point into them."""
+
+    def __init__(self):
+        self.keep: List[object] = []
+
+
+def geometry_to_c(g: Geometry, hold: _Holder) -> CGeometry:
+    cg = CGeometry()
+    vox = np.ascontiguousarray(g.voxels, dtype=np.uint8)
+    hold.keep.append(vox)
+    for a in range(3):
+        cg.N[a] = vox.shape[a]
+        cg.global_size[a] = (g.global_size[a] if g.global_size else vox.shape[a] * g.h)
+    cg.h = g.h
+    nd = len(g.domain_names)
+    if nd > MAX_DOMAINS:
+        raise ValueError("too many domains")
+    cg.num_domains = nd
+    for i, n in enumerate(g.domain_names):
+        _setname(cg.domain_names[i], n)
+        cg.domain_indices[i] = i
+    conn = np.asarray(g.domain_connections, dtype=np.uint8).reshape(nd, nd)
+    for i in range(nd):
+        for j in range(nd):
+            cg.domain_connections[i * nd + j] = int(conn[i, j])
+    cg.voxels = vox.ctypes.data_as(C.POINTER(C.c_uint8))
+    if len(g.boundaries) > MAX_BOUNDARIES:
+        raise ValueError("too many boundaries")
+    cg.num_boundaries = len(g.boundaries)
+    for i, b in enumerate(g.boundaries):
+        arr = np.ascontiguousarray(b.pairs, dtype=np.uint16).reshape(-1, 6)
+        hold.keep.append(arr)
+        _setname(cg.boundary_names[i], b.name)
+        cg.boundary_types[i] = BOUNDARY_DISCRETE if b.type == "discrete" else BOUNDARY_DENSITY
+        cg.boundary_sizes[i] = arr.shape[0]
+        cg.boundaries[i] = arr.ctypes.data_as(C.POINTER(C.c_uint16))
+    return cg
+
+
+def _channel_to_c(dst: CChannelModel, m: ChannelModel):
+    for k in ("Kd_open", "k_min_open", "k_max_open", "n_open", "Kd_close", "k_min_close",
+              "k_max_close", "n_close", "i_unitary"):
+        setattr(dst, k, float(getattr(m, k)))
+    _setfield(dst, "boundary", m.boundary)
+    _setfield(dst, "species", m.species)
+
+
+def model_to_c(m: SpeciesModel) -> CSpeciesModel:
+    cm = CSpeciesModel()
+    nd, ns = len(m.domain_names), len(m.species_names)
+    if nd > MAX_DOMAINS or ns > MAX_SPECIES:
+        raise ValueError("too many domains or species")
+    cm.num_domains, cm.num_species = nd, ns
+    for i, n in enumerate(m.domain_names):
+        _setname(cm.domain_names[i], n)
+    for i, n in enumerate(m.species_names):
+        _setname(cm.species_names[i], n)
+    init = np.asarray(m.init, dtype=np.float64).reshape(nd, ns)
+    for d in range(nd):
+        cm.num_diffusive[d] = len(m.diffusive[d])
+        for k, (sp, sg) in enumerate(zip(m.diffusive[d], m.sigma[d])):
+            cm.diffusive[d][k] = sp
+            cm.sigma[d][k] = sg
+        for s in range(ns):
+            cm.init[d][s] = init[d, s]
+        cm.num_buffers[d] = len(m.buffers[d])
+        for k, b in enumerate(m.buffers[d]):
+            cm.k_on[d][k], cm.k_off[d][k], cm.tot[d][k] = b.k_on, b.k_off, b.tot
+            cm.buffer_species[d][k][0] = b.bound
+            cm.buffer_species[d][k][1] = b.free
+    if m.ryr is not None:
+        cm.use_ryr = 1
+        _channel_to_c(cm.ryr, m.ryr)
+    if m.p_ryr is not None:
+        cm.use_p_ryr = 1
+        _channel_to_c(cm.p_ryr, m.p_ryr)
+    if m.serca is not None:
+        cm.use_serca = 1
+        cm.serca.v_cyt_to_A_sr = m.serca.v_cyt_to_A_sr
+        cm.serca.density = m.serca.density
+        cm.serca.scale = m.serca.scale
+        _setfield(cm.serca, "boundary", m.serca.boundary)
+        # the reference takes the SERCA species from /ryr/species
+        _setfield(cm.serca, "species", m.ryr.species if m.ryr is not None else m.serca.species)
+    for a in range(3):
+        cm.conv[a] = m.conv[a]
+    cm.num_fixed = len(m.fixed)
+    for i, (d, s) in enumerate(m.fixed):
+        cm.fixed[2 * i], cm.fixed[2 * i + 1] = d, s
+    return cm
+
+
+def options_to_c(o: Options) -> COptions:
+    co = COptions()
+    host_lib().cs_options_defaults(C.byref(co))
+    co.tstop, co.h, co.dt, co.dt_save = o.tstop, o.h, o.dt, o.dt_save
+    co.dt_update_react, co.dt_update_stoch, co.tabort = o.dt_update_react, o.dt_update_stoch, o.tabort
+    co.force_dirichlet = 1 if o.force_dirichlet else 0
+    if o.seed is not None:
+        co.seed_given, co.seed = 1, int(o.seed) & 0xFFFFFFFF
+    if len(o.open) > MAX_NAMED or len(o.t_close) > MAX_NAMED:
+        raise ValueError("too many -O / -C entries")
+    co.num_open = len(o.open)
+    for i, (name, idx) in enumerate(o.open.items()):
+        _setfield(co.open[i], "name", name)
+        idx = list(idx)
+        if len(idx) > MAX_OPEN:
+            raise ValueError("too many open indices")
+        co.open[i].count = len(idx)
+        for j, v in enumerate(idx):
+            co.open[i].idx[j] = int(v)
+    co.num_t_close = len(o.t_close)
+    for i, (name, t) in enumerate(o.t_close.items()):
+        _setfield(co.t_close[i], "name", name)
+        co.t_close[i].time = float(t)
+    co.precision = PREC_F32 if o.precision == "f32" else PREC_F64
+    co.rng = RNG_MT if o.rng == "mt" else RNG_PHILOX
+    return co
+
+
+class Setup:
+    """cs_setup_build result: the derived problem plus host metadata."""
+
+    def __init__(self, geom: Geometry, model: SpeciesModel, opts: Options):
+        self._hold = _Holder()
+        self.cgeom = geometry_to_c(geom, self._hold)
+        self.cmodel = model_to_c(model)
+        self.copts = options_to_c(opts)
+        err = C.create_string_buffer(512)
+        ptr = C.POINTER(CSetup)()
+        rc = host_lib().cs_setup_build(C.byref(self.cgeom), C.byref(self.cmodel), C.byref(self.copts),
+                                       C.byref(ptr), err, 512)
+        if rc != 0:
+            raise CSError(err.value.decode(errors="replace"))
+        self.ptr = ptr
+        self.s = ptr.contents
+
+    @property
+    def problem(self) -> CProblem:
+        return self.s.problem
+
+    @property
+    def n(self) -> Tuple[int, int, int]:
+        return tuple(int(v) for v in self.s.problem.n)
+
+    @property
+    def dt(self) -> float:
+        return self.s.problem.dt
+
+    def mt_uniforms(self, first: int, nsteps: int) -> np.ndarray:
+        cnt = C.c_uint64()
+        host_lib().cs_setup_mt_uniforms(self.ptr, first, nsteps, None, C.byref(cnt))
+        out = np.empty(cnt.value, dtype=np.float64)
+        host_lib().cs_setup_mt_uniforms(self.ptr, first, nsteps,
+                                        out.ctypes.data_as(C.POINTER(C.c_double)), C.byref(cnt))
+        return out
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            host_lib().cs_setup_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Simulation:
+    """A resident calcium_sparks domain on the current GPU."""
+
+    def __init__(self, geom: Geometry, model: SpeciesModel, opts: Options):
+        self.setup = Setup(geom, model, opts)
+        self.opts = opts
+        self.ctx = C.c_void_p()
+        _check(device_lib().cs_create(C.byref(self.setup.s.problem), C.byref(self.ctx)), "cs_create")
+        self.time_ind = 1          # next time index (the reference starts at 1)
+        self.ns = int(self.setup.s.num_species)
+        self.nd = int(self.setup.s.num_domains)
+
+    @property
+    def n(self):
+        return self.setup.n
+
+    @property
+    def dt(self):
+        return self.setup.dt
+
+    @property
+    def problem(self) -> CProblem:
+        return self.setup.problem
+
+    def step(self, nsteps: int = 1):
+        lib = device_lib()
+        if self.setup.problem.rng == RNG_MT:
+            u = self.setup.mt_uniforms(self.time_ind, nsteps)
+            _check(lib.cs_step_uniforms(self.ctx, self.time_ind, nsteps,
+                                        u.ctypes.data_as(C.POINTER(C.c_double)), len(u)), "cs_step_uniforms")
+        else:
+            _check(lib.cs_step(self.ctx, self.time_ind, nsteps), "cs_step")
+        self.time_ind += nsteps
+
+    def synchronize(self):
+        _check(device_lib().cs_synchronize(self.ctx), "cs_synchronize")
+
+    def species(self, s: int) -> np.ndarray:
+        out = np.empty(self.n, dtype=np.float64)
+        _check(device_lib().cs_download_species(self.ctx, s, out.ctypes.data_as(C.POINTER(C.c_double))),
+               "cs_download_species")
+        return out
+
+    def set_species(self, s: int, values: np.ndarray):
+        arr = np.ascontiguousarray(values, dtype=np.float64).reshape(self.n)
+        _check(device_lib().cs_upload_species(self.ctx, s, arr.ctypes.data_as(C.POINTER(C.c_double))),
+               "cs_upload_species")
+
+    def stats(self):
+        k = self.nd * self.ns
+        mn, mx, tot = (np.empty(k), np.empty(k), np.empty(k))
+        ptr = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+        _check(device_lib().cs_reduce_stats(self.ctx, ptr(mn), ptr(mx), ptr(tot)), "cs_reduce_stats")
+        return mn.reshape(self.nd, self.ns), mx.reshape(self.nd, self.ns), tot.reshape(self.nd, self.ns)
+
+    def channel_states(self, flux: int) -> np.ndarray:
+        n = self.setup.problem.fluxes[flux].num_pairs
+        out = np.zeros(max(n, 1), dtype=np.uint8)
+        _check(device_lib().cs_get_channel_states(self.ctx, flux, out.ctypes.data_as(C.POINTER(C.c_uint8))),
+               "cs_get_channel_states")
+        return out[:n]
+
+    def set_channel_states(self, flux: int, states):
+        arr = np.ascontiguousarray(states, dtype=np.uint8)
+        _check(device_lib().cs_set_channel_states(self.ctx, flux, arr.ctypes.data_as(C.POINTER(C.c_uint8))),
+               "cs_set_channel_states")
+
+    def time_steps(self, nsteps: int):
+        """Time nsteps steps with HIP events; returns (total_ms, mean stencil kernel ms)."""
+        tot, ker = C.c_double(), C.c_double()
+        _check(device_lib().cs_time_steps(self.ctx, self.time_ind, nsteps, C.byref(tot), C.byref(ker)),
+               "cs_time_steps")
+        self.time_ind += nsteps
+        return tot.value, ker.value
+
+    def close(self):
+        if getattr(self, "ctx", None) and self.ctx.value:
+            device_lib().cs_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+        if getattr(self, "setup", None) is not None:
+            self.setup.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_info() -> str:
+    dev = C.c_int()
+    buf = C.create_string_buffer(256)
+    _check(device_lib().cs_device_info(C.byref(dev), buf, 256), "cs_device_info")
+    return f"device {dev.value}: {buf.value.decode()}"

@@ -1,0 +1,1056 @@
+/*
+ * csparks.hip -- MI355X (gfx950) time stepping for calcium_sparks.
+ *
+ * Device layout.  Every species is one array in HBM indexed
+ * (z*ny + y)*nx + x, x fastest, where (x, y, z) are the reference's axes
+ * 0, 1, 2; the species arrays are stacked [species][z][y][x].  Two such
+ * stacks ping-pong: a step reads stack `cur` and writes the complete next
+ * state into the other, which replaces the reference's du buffer and its
+ * apply_du copy (species.c:1080-1121).  Domain ids are one byte per voxel:
+ * bits 0-2 hold the domain, bit 7 is set when all six in-grid neighbours
+ * share that domain (the reference's stencil_test, kernel_generic.cpp:
+ * 125-173, inverted).
+ *
+ * One step = three launches on the context stream:
+ *   k_gate    (every stochastic substep) one block updates the channel
+ *             states from Ca at the channel mouths (evaluate_RyR_
+ *             stochastically, boundaryfluxes.c:295-359);
+ *   k_stencil the memory-bound pass: one thread per (x,y) column marches
+ *             z with a register ring of planes z-1, z, z+1 and writes every
+ *             species of the next state: 7-point stencil (species.c:1501-
+ *             1650), buffer reactions (reaction_SIMD, kernel_generic.cpp:
+ *             397-509), Dirichlet faces and fixed domain species
+ *             (species.c:1037-1077);
+ *   k_flux    one thread per membrane voxel gathers every boundary-pair
+ *             flux touching it (species.c:1704-1807) -- a gather, so no
+ *             floating-point atomics and a fixed summation order.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "csparks.h"
+#include "cs_physics.h"
+
+namespace {
+
+constexpr int MAXS = CS_MAX_SPECIES;
+constexpr int MAXD = CS_MAX_DOMAINS;
+constexpr int MAXF = CS_MAX_FLUXES;
+constexpr int BX = 64;          /* one wavefront along x                  */
+constexpr int BY = 4;           /* four wavefronts per block along y      */
+constexpr int GATE_THREADS = 256;
+constexpr int FLUX_THREADS = 256;
+constexpr int STAT_THREADS = 256;
+constexpr int STAT_BLOCKS = 512;
+
+thread_local std::string g_err;
+
+int set_err(int code, const char* fmt, ...)
+{
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return set_err(CS_EDEVICE, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+struct FluxParams {
+  uint32_t kind, species, discrete, inner;
+  uint32_t npairs, pair_off, state_off, stochastic;
+  uint32_t group, uniform_off;   /* gating group, offset among draws     */
+  double   half_1mK0;            /* (1-K_0)/2 for RyR                    */
+  double   density;              /* SERCA pumps per nm^2                 */
+  double   p[CS_NUM_RYR_PARAMS]; /* reference parameter vector           */
+};
+
+template <typename T>
+struct DevParams {
+  T coef[MAXS][MAXD * MAXD];
+  T kon[MAXS][MAXD], koff[MAXS][MAXD], tot[MAXS][MAXD];
+  T init[MAXD][MAXS];
+  T dt_react;
+  uint8_t bsp0[MAXS][MAXD], bsp1[MAXS][MAXD];
+  uint32_t nb;
+  uint32_t fixed_mask[MAXD];
+  uint32_t dirichlet;
+  uint32_t nflux, ndiscrete;
+  double dt, h;
+  FluxParams flux[MAXF];
+};
+
+struct Geo {
+  int nx, ny, nz;
+  uint64_t plane, nv;
+};
+
+/* ------------------------------------------------------------ kernels */
+
+template <typename T, int NS>
+__global__ void __launch_bounds__(BX * BY)
+k_stencil(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__ ids,
+          const DevParams<T>* __restrict__ P, Geo g, uint32_t dmask, uint32_t react, int zchunk)
+{
+  __shared__ T s_coef[NS][MAXD * MAXD];
+  __shared__ T s_kon[MAXS][MAXD], s_koff[MAXS][MAXD], s_tot[MAXS][MAXD];
+  __shared__ T s_init[MAXD][NS];
+  __shared__ uint8_t s_b0[MAXS][MAXD], s_b1[MAXS][MAXD];
+  __shared__ uint32_t s_fixed[MAXD];
+
+  const int tid = threadIdx.y * BX + threadIdx.x;
+  for (int i = tid; i < NS * MAXD * MAXD; i += BX * BY) s_coef[i / (MAXD * MAXD)][i % (MAXD * MAXD)] = P->coef[i / (MAXD * MAXD)][i % (MAXD * MAXD)];
+  for (int i = tid; i < MAXS * MAXD; i += BX * BY) {
+    const int b = i / MAXD, d = i % MAXD;
+    s_kon[b][d] = P->kon[b][d];
+    s_koff[b][d] = P->koff[b][d];
+    s_tot[b][d] = P->tot[b][d];
+    s_b0[b][d] = P->bsp0[b][d];
+    s_b1[b][d] = P->bsp1[b][d];
+  }
+  for (int i = tid; i < MAXD * NS; i += BX * BY) s_init[i / NS][i % NS] = P->init[i / NS][i % NS];
+  if (tid < MAXD) s_fixed[tid] = P->fixed_mask[tid];
+  __syncthreads();
+
+  const int x = blockIdx.x * BX + threadIdx.x;
+  const int y = blockIdx.y * BY + threadIdx.y;
+  if (x >= g.nx || y >= g.ny) return;
+  const int z0 = blockIdx.z * zchunk;
+  const int z1 = min(g.nz, z0 + zchunk);
+  if (z0 >= z1) return;
+
+  const int nb = react ? (int)P->nb : 0;
+  const T dtr = P->dt_react;
+  const bool dirichlet = P->dirichlet != 0;
+  /* clamped neighbour offsets: a missing neighbour reads the voxel
+   * itself, so its (u_n - u) term vanishes -- the no-flux outer faces */
+  const int64_t oxm = x > 0 ? -1 : 0, oxp = x + 1 < g.nx ? 1 : 0;
+  const int64_t oym = y > 0 ? -(int64_t)g.nx : 0, oyp = y + 1 < g.ny ? (int64_t)g.nx : 0;
+  const bool xyface = x == 0 || y == 0 || x + 1 == g.nx || y + 1 == g.ny;
+  const uint64_t col = (uint64_t)y * g.nx + x;
+
+  T um[NS], uc[NS], up[NS];
+  {
+    const uint64_t v = (uint64_t)z0 * g.plane + col;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      uc[s] = u[s * g.nv + v];
+      um[s] = z0 > 0 ? u[s * g.nv + v - g.plane] : uc[s];
+    }
+  }
+  for (int z = z0; z < z1; z++) {
+    const uint64_t v = (uint64_t)z * g.plane + col;
+    const bool ztop = z + 1 < g.nz;
+#pragma unroll
+    for (int s = 0; s < NS; s++) up[s] = ztop ? u[s * g.nv + v + g.plane] : uc[s];
+
+    const uint8_t id = ids[v];
+    const int d = id & 7;
+    const bool uni = (id & 0x80) != 0;
+    int dxm = d, dxp = d, dym = d, dyp = d, dzm = d, dzp = d;
+    if (!uni) {
+      dxm = ids[v + oxm] & 7; dxp = ids[v + oxp] & 7;
+      dym = ids[v + oym] & 7; dyp = ids[v + oyp] & 7;
+      dzm = z > 0 ? ids[v - g.plane] & 7 : d;
+      dzp = ztop ? ids[v + g.plane] & 7 : d;
+    }
+    T nw[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      T acc = uc[s];
+      if ((dmask >> s) & 1u) {
+        const T* us = u + s * g.nv;
+        const T c = uc[s];
+        const T exm = us[v + oxm] - c, exp_ = us[v + oxp] - c;
+        const T eym = us[v + oym] - c, eyp = us[v + oyp] - c;
+        const T ezm = um[s] - c, ezp = up[s] - c;
+        const T* C = s_coef[s] + d * MAXD;
+        if (uni) {
+          acc = c + C[d] * ((ezm + ezp) + (eym + eyp) + (exm + exp_));
+        } else {
+          acc = c + ((C[dzm] * ezm + C[dzp] * ezp) + (C[dym] * eym + C[dyp] * eyp) +
+                     (C[dxm] * exm + C[dxp] * exp_));
+        }
+      }
+      nw[s] = acc;
+    }
+    /* buffers in the reference's order: the bound species first, then the
+     * free one; a species not updated earlier this step is overwritten
+     * from u rather than accumulated, as reaction_SIMD does */
+    for (int b = 0; b < nb; b++) {
+      const int b0 = s_b0[b][d], b1 = s_b1[b][d];
+      T B0 = uc[0], B1 = uc[0];
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        if (s == b0) B0 = uc[s];
+        if (s == b1) B1 = uc[s];
+      }
+      const T J = s_kon[b][d] * (s_tot[b][d] - B0) * B1 - s_koff[b][d] * B0;
+      const T inc = dtr * J;
+#pragma unroll
+      for (int s = 0; s < NS; s++)
+        if (s == b0) nw[s] = (((dmask >> s) & 1u) ? nw[s] : uc[s]) + inc;
+#pragma unroll
+      for (int s = 0; s < NS; s++)
+        if (s == b1) nw[s] = (((dmask >> s) & 1u) ? nw[s] : uc[s]) - inc;
+    }
+    if (dirichlet && (xyface || z == 0 || !ztop)) nw[0] = s_init[d][0];
+    const uint32_t fm = s_fixed[d];
+    if (fm) {
+#pragma unroll
+      for (int s = 0; s < NS; s++)
+        if ((fm >> s) & 1u) nw[s] = s_init[d][s];
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      un[s * g.nv + v] = nw[s];
+      um[s] = uc[s];
+      uc[s] = up[s];
+    }
+  }
+}
+
+/* Channel gating: one block, channels strided over threads, fluxes in
+ * gating order.  Ca is read from species slot 0 at the channel's cleft
+ * voxel (species.c:2569). */
+template <typename T>
+__global__ void __launch_bounds__(GATE_THREADS)
+k_gate(const T* __restrict__ u, const uint32_t* __restrict__ pairs, uint8_t* __restrict__ states,
+       const DevParams<T>* __restrict__ P, uint64_t ti_arg, const uint64_t* __restrict__ ti_ptr,
+       uint32_t D, uint32_t seed, uint32_t instance, const double* __restrict__ uniforms,
+       uint64_t uniform_base)
+{
+  const uint64_t ti = (ti_ptr ? *ti_ptr : 0) + ti_arg;
+  const double t = (double)ti * P->dt;
+  const double dts = P->dt * D;
+  const uint32_t nf = P->ndiscrete;
+  for (uint32_t f = 0; f < nf; f++) {
+    const FluxParams& F = P->flux[f];
+    const double* p = F.p;
+    for (uint32_t ch = threadIdx.x; ch < F.npairs; ch += blockDim.x) {
+      uint8_t st = states[F.state_off + ch];
+      if (!F.stochastic) {
+        if (p[8] <= t + dts) st = 0;
+      } else {
+        const uint32_t to = pairs[2 * (F.pair_off + ch) + 1];
+        const double c0 = (double)u[to];
+        const double kc = cs_rate_close(c0, p);
+        const double ko = cs_rate_open(c0, p);
+        const double r = uniforms ? uniforms[uniform_base + F.uniform_off + ch]
+                                  : cs_gate_uniform(seed, instance, ti, F.group, ch);
+        if (st == 0) { if (r <= dts * ko) st = 1; }
+        else if (st == 1) { if (r <= dts * kc) st = 0; }
+      }
+      states[F.state_off + ch] = st;
+    }
+  }
+}
+
+/* Boundary fluxes gathered per (species, voxel) row. */
+template <typename T>
+__global__ void __launch_bounds__(FLUX_THREADS)
+k_flux(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__ ids,
+       const uint32_t* __restrict__ row_voxel, const uint8_t* __restrict__ row_species,
+       const uint32_t* __restrict__ row_start, const uint32_t* __restrict__ ent_pair,
+       const uint8_t* __restrict__ ent_flux, const int8_t* __restrict__ ent_sign,
+       const uint32_t* __restrict__ pairs, const uint8_t* __restrict__ states,
+       const DevParams<T>* __restrict__ P, Geo g, uint32_t nrows, uint32_t fluxmask)
+{
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const uint32_t v = row_voxel[r];
+  const uint32_t s = row_species[r];
+  const int d = ids[v] & 7;
+  if ((P->fixed_mask[d] >> s) & 1u) return;
+  if (s == 0 && P->dirichlet) {
+    const uint32_t x = v % g.nx, y = (v / g.nx) % g.ny, z = (uint32_t)(v / g.plane);
+    if (x == 0 || y == 0 || z == 0 || x + 1 == (uint32_t)g.nx || y + 1 == (uint32_t)g.ny ||
+        z + 1 == (uint32_t)g.nz)
+      return;
+  }
+  const T* us = u + s * g.nv;
+  const double dt = P->dt, h = P->h;
+  double sum = 0.0;
+  bool any = false;
+  for (uint32_t e = row_start[r]; e < row_start[r + 1]; e++) {
+    const uint32_t f = ent_flux[e];
+    if (!((fluxmask >> f) & 1u)) continue;
+    const FluxParams& F = P->flux[f];
+    const uint32_t p = ent_pair[e];
+    if (F.discrete && !states[F.state_off + p]) continue;
+    const uint32_t from = pairs[2 * (F.pair_off + p)], to = pairs[2 * (F.pair_off + p) + 1];
+    const T u0 = us[from], u1 = us[to];
+    double J;
+    if (F.kind == CS_FLUX_RYR)
+      J = (double)((u0 - u1) * (T)F.half_1mK0);
+    else
+      J = cs_serca_flux_d(dt, h, (double)u0, (double)u1, F.density);
+    sum += ent_sign[e] > 0 ? J : -J;
+    any = true;
+  }
+  if (any) {
+    T* o = un + s * g.nv + v;
+    *o = (T)((double)*o + sum);
+  }
+}
+
+/* Per-(species, domain) min / max / sum(u*dV): block partials in a fixed
+ * order, then one pass over the partials (no atomics, reproducible). */
+template <typename T>
+__device__ inline void block_reduce3(double (&mn)[MAXD], double (&mx)[MAXD], double (&sm)[MAXD],
+                                     double* sh /* [3][MAXD][STAT_THREADS/64] */)
+{
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nw = STAT_THREADS / 64;
+#pragma unroll
+  for (int d = 0; d < MAXD; d++) {
+    for (int off = 32; off > 0; off >>= 1) {
+      mn[d] = fmin(mn[d], __shfl_down(mn[d], off));
+      mx[d] = fmax(mx[d], __shfl_down(mx[d], off));
+      sm[d] += __shfl_down(sm[d], off);
+    }
+    if (lane == 0) {
+      sh[(0 * MAXD + d) * nw + wave] = mn[d];
+      sh[(1 * MAXD + d) * nw + wave] = mx[d];
+      sh[(2 * MAXD + d) * nw + wave] = sm[d];
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(STAT_THREADS)
+k_stats(const T* __restrict__ u, const uint8_t* __restrict__ ids, Geo g, uint32_t ns, double dV,
+        double* __restrict__ partial /* [grid][ns][MAXD][3] */)
+{
+  __shared__ double sh[3 * MAXD * (STAT_THREADS / 64)];
+  const int nw = STAT_THREADS / 64;
+  for (uint32_t s = 0; s < ns; s++) {
+    double mn[MAXD], mx[MAXD], sm[MAXD];
+#pragma unroll
+    for (int d = 0; d < MAXD; d++) { mn[d] = FLT_MAX; mx[d] = FLT_MIN; sm[d] = 0.0; }
+    const T* us = u + s * g.nv;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.nv;
+         v += (uint64_t)gridDim.x * blockDim.x) {
+      const int dv = ids[v] & 7;
+      const double val = (double)us[v];
+#pragma unroll
+      for (int d = 0; d < MAXD; d++)
+        if (d == dv) {
+          mn[d] = val < mn[d] ? val : mn[d];
+          mx[d] = val > mx[d] ? val : mx[d];
+          sm[d] += val * dV;
+        }
+    }
+    block_reduce3<T>(mn, mx, sm, sh);
+    __syncthreads();
+    if (threadIdx.x < MAXD) {
+      const int d = threadIdx.x;
+      double a = sh[(0 * MAXD + d) * nw], b = sh[(1 * MAXD + d) * nw], c = sh[(2 * MAXD + d) * nw];
+      for (int w = 1; w < nw; w++) {
+        a = fmin(a, sh[(0 * MAXD + d) * nw + w]);
+        b = fmax(b, sh[(1 * MAXD + d) * nw + w]);
+        c += sh[(2 * MAXD + d) * nw + w];
+      }
+      double* o = partial + (((uint64_t)blockIdx.x * ns + s) * MAXD + d) * 3;
+      o[0] = a; o[1] = b; o[2] = c;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_stats_final(const double* __restrict__ partial, uint32_t nblocks, uint32_t ns,
+                              double* __restrict__ out /* [ns][MAXD][3] */)
+{
+  const uint32_t i = threadIdx.x;
+  if (i >= ns * MAXD) return;
+  double a = FLT_MAX, b = FLT_MIN, c = 0.0;
+  for (uint32_t k = 0; k < nblocks; k++) {
+    const double* p = partial + ((uint64_t)k * ns * MAXD + i) * 3;
+    a = p[0] < a ? p[0] : a;
+    b = p[1] > b ? p[1] : b;
+    c += p[2];
+  }
+  out[3 * i + 0] = a; out[3 * i + 1] = b; out[3 * i + 2] = c;
+}
+
+/* Fill every voxel with init[domain][species]. */
+template <typename T>
+__global__ void k_fill_init(T* __restrict__ u, const uint8_t* __restrict__ ids,
+                            const DevParams<T>* __restrict__ P, Geo g, uint32_t ns)
+{
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.nv;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    const int d = ids[v] & 7;
+    for (uint32_t s = 0; s < ns; s++) u[s * g.nv + v] = P->init[d][s];
+  }
+}
+
+}  // namespace
+
+/* ------------------------------------------------------------ context */
+
+struct cs_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t precision = CS_PREC_F64;
+  uint32_t ns = 0, nd = 0, nf = 0, ndiscrete = 0;
+  uint32_t n[3] = {0, 0, 0};
+  Geo geo{};
+  size_t esize = 8;
+  void* d_u[2] = {nullptr, nullptr};
+  int cur = 0;
+  uint8_t* d_ids = nullptr;
+  void* d_params = nullptr;
+  /* boundary fluxes */
+  uint32_t* d_pairs = nullptr;
+  uint8_t* d_states = nullptr;
+  uint32_t nrows = 0;
+  uint32_t* d_row_voxel = nullptr;
+  uint8_t* d_row_species = nullptr;
+  uint32_t* d_row_start = nullptr;
+  uint32_t* d_ent_pair = nullptr;
+  uint8_t* d_ent_flux = nullptr;
+  int8_t* d_ent_sign = nullptr;
+  /* gating */
+  double* d_uniforms = nullptr;
+  uint64_t uniform_cap = 0;
+  uint32_t stoch_per_gate = 0;
+  /* reductions */
+  double* d_partial = nullptr;
+  double* d_stats = nullptr;
+  /* host copies */
+  cs_problem prob{};
+  std::vector<uint32_t> state_off, npairs;
+  std::vector<uint8_t> init_states;
+  uint32_t substeps[MAXS] = {0};
+  uint32_t flux_substep[MAXF] = {0};
+  int zchunk = 1, zsplit = 1;
+};
+
+namespace {
+
+void free_ctx(cs_context* c)
+{
+  if (!c) return;
+  hipSetDevice(c->device);
+  void* ptrs[] = {c->d_u[0], c->d_u[1], c->d_ids, c->d_params, c->d_pairs, c->d_states,
+                  c->d_row_voxel, c->d_row_species, c->d_row_start, c->d_ent_pair,
+                  c->d_ent_flux, c->d_ent_sign, c->d_uniforms, c->d_partial, c->d_stats};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+inline uint32_t ref_to_dev(uint32_t i, const uint32_t n[3])
+{
+  const uint32_t z = i % n[2], y = (i / n[2]) % n[1], x = i / (n[2] * n[1]);
+  return (z * n[1] + y) * n[0] + x;
+}
+
+template <typename T>
+void fill_params(DevParams<T>& dp, const cs_problem* p, const cs_context* c)
+{
+  memset(&dp, 0, sizeof dp);
+  for (int s = 0; s < MAXS; s++)
+    for (int i = 0; i < MAXD * MAXD; i++) dp.coef[s][i] = (T)p->coef[s][i];
+  for (int b = 0; b < MAXS; b++)
+    for (int d = 0; d < MAXD; d++) {
+      dp.kon[b][d] = (T)p->k_on[b][d];
+      dp.koff[b][d] = (T)p->k_off[b][d];
+      dp.tot[b][d] = (T)p->tot[b][d];
+      dp.bsp0[b][d] = p->bsp0[b][d];
+      dp.bsp1[b][d] = p->bsp1[b][d];
+    }
+  for (int d = 0; d < MAXD; d++)
+    for (int s = 0; s < MAXS; s++) {
+      dp.init[d][s] = (T)p->init[d][s];
+      if (p->fixed[d][s]) dp.fixed_mask[d] |= 1u << s;
+    }
+  dp.dt_react = (T)p->dt_react;
+  dp.nb = p->num_buffers;
+  dp.dirichlet = p->force_dirichlet;
+  dp.nflux = p->num_fluxes;
+  dp.ndiscrete = c->ndiscrete;
+  dp.dt = p->dt;
+  dp.h = p->h;
+  uint32_t poff = 0, soff = 0, uoff = 0, group = 0;
+  for (uint32_t f = 0; f < p->num_fluxes; f++) {
+    const cs_flux_desc& F = p->fluxes[f];
+    FluxParams& o = dp.flux[f];
+    o.kind = F.kind;
+    o.species = F.species;
+    o.discrete = F.discrete;
+    o.inner = F.inner;
+    o.npairs = F.num_pairs;
+    o.pair_off = poff;
+    o.state_off = soff;
+    poff += F.num_pairs;
+    if (F.discrete) {
+      soff += F.num_pairs;
+      o.group = group++;
+      o.stochastic = !(F.params[8] > 0);
+      o.uniform_off = uoff;
+      if (o.stochastic) uoff += F.num_pairs;
+    }
+    memcpy(o.p, F.params, sizeof o.p);
+    o.half_1mK0 = F.kind == CS_FLUX_RYR ? (1 - cs_ryr_k0(p->dt, p->h, F.params[6])) / 2 : 0.0;
+    o.density = F.kind == CS_FLUX_SERCA ? F.params[1] : 0.0;
+  }
+}
+
+int validate(const cs_problem* p)
+{
+  if (!p) return set_err(CS_EINVAL, "problem is NULL");
+  if (p->n[0] < 1 || p->n[1] < 1 || p->n[2] < 1)
+    return set_err(CS_EINVAL, "grid must have at least one voxel per axis");
+  const uint64_t nv = (uint64_t)p->n[0] * p->n[1] * p->n[2];
+  if (nv > 0xFFFFFFFFull) return set_err(CS_ERANGE, "more than 2^32 voxels");
+  if (p->num_species < 1 || p->num_species > MAXS)
+    return set_err(CS_EINVAL, "num_species %u outside 1..%d", p->num_species, MAXS);
+  if (p->num_domains < 1 || p->num_domains > MAXD)
+    return set_err(CS_EINVAL, "num_domains %u outside 1..%d", p->num_domains, MAXD);
+  if (p->precision != CS_PREC_F64 && p->precision != CS_PREC_F32)
+    return set_err(CS_EINVAL, "unknown precision %u", p->precision);
+  if (!p->domain_ids) return set_err(CS_EINVAL, "domain_ids is NULL");
+  if (p->num_buffers > MAXS) return set_err(CS_EINVAL, "too many buffers");
+  if (p->num_fluxes > MAXF) return set_err(CS_EINVAL, "too many fluxes");
+  if (p->reaction_substep < 1 || p->stochastic_substep < 1)
+    return set_err(CS_EINVAL, "substeps must be >= 1");
+  if (!(p->dt > 0) || !(p->h > 0)) return set_err(CS_EINVAL, "dt and h must be positive");
+  bool seen_density = false;
+  for (uint32_t f = 0; f < p->num_fluxes; f++) {
+    const cs_flux_desc& F = p->fluxes[f];
+    if (F.kind != CS_FLUX_RYR && F.kind != CS_FLUX_SERCA)
+      return set_err(CS_EINVAL, "flux %u: unknown kind", f);
+    if (F.species >= p->num_species) return set_err(CS_EINVAL, "flux %u: bad species", f);
+    if (F.substep < 1) return set_err(CS_EINVAL, "flux %u: substep must be >= 1", f);
+    if (F.num_pairs && !F.pairs) return set_err(CS_EINVAL, "flux %u: pairs is NULL", f);
+    if (F.discrete) {
+      if (seen_density) return set_err(CS_EINVAL, "discrete fluxes must come first");
+      if (F.kind != CS_FLUX_RYR) return set_err(CS_EINVAL, "flux %u: discrete flux must be RyR", f);
+      if (F.num_pairs && !F.init_states) return set_err(CS_EINVAL, "flux %u: init_states is NULL", f);
+    } else {
+      seen_density = true;
+    }
+    for (uint32_t i = 0; i < 2 * F.num_pairs; i++)
+      if (F.pairs[i] >= nv) return set_err(CS_ERANGE, "flux %u: pair voxel out of range", f);
+  }
+  for (uint64_t v = 0; v < nv; v++)
+    if (p->domain_ids[v] >= p->num_domains)
+      return set_err(CS_ERANGE, "domain id %u at voxel %llu out of range", p->domain_ids[v],
+                     (unsigned long long)v);
+  for (uint32_t s = 0; s < p->num_species; s++)
+    if (p->substeps[s] > 255) return set_err(CS_EINVAL, "substep too large");
+  return CS_OK;
+}
+
+template <typename T>
+int upload_params(cs_context* c)
+{
+  DevParams<T> dp;
+  fill_params<T>(dp, &c->prob, c);
+  if (!c->d_params) HIP_TRY(hipMalloc(&c->d_params, sizeof(DevParams<T>)));
+  HIP_TRY(hipMemcpyAsync(c->d_params, &dp, sizeof dp, hipMemcpyHostToDevice, c->stream));
+  return CS_OK;
+}
+
+template <typename T>
+int fill_init(cs_context* c)
+{
+  const int blocks = (int)std::min<uint64_t>(4096, (c->geo.nv + 255) / 256);
+  for (int b = 0; b < 2; b++)
+    hipLaunchKernelGGL(k_fill_init<T>, dim3(blocks), dim3(256), 0, c->stream, (T*)c->d_u[b],
+                       c->d_ids, (const DevParams<T>*)c->d_params, c->geo, c->ns);
+  HIP_TRY(hipGetLastError());
+  if (c->ndiscrete && !c->init_states.empty())
+    HIP_TRY(hipMemcpyAsync(c->d_states, c->init_states.data(), c->init_states.size(),
+                           hipMemcpyHostToDevice, c->stream));
+  c->cur = 0;
+  return CS_OK;
+}
+
+template <typename T, int NS>
+void launch_stencil_ns(cs_context* c, uint32_t dmask, uint32_t react)
+{
+  const Geo& g = c->geo;
+  dim3 grid((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, c->zsplit);
+  hipLaunchKernelGGL((k_stencil<T, NS>), grid, dim3(BX, BY), 0, c->stream,
+                     (const T*)c->d_u[c->cur], (T*)c->d_u[c->cur ^ 1], c->d_ids,
+                     (const DevParams<T>*)c->d_params, g, dmask, react, c->zchunk);
+}
+
+template <typename T>
+void launch_stencil(cs_context* c, uint32_t dmask, uint32_t react)
+{
+  switch (c->ns) {
+    case 1: launch_stencil_ns<T, 1>(c, dmask, react); break;
+    case 2: launch_stencil_ns<T, 2>(c, dmask, react); break;
+    case 3: launch_stencil_ns<T, 3>(c, dmask, react); break;
+    case 4: launch_stencil_ns<T, 4>(c, dmask, react); break;
+    case 5: launch_stencil_ns<T, 5>(c, dmask, react); break;
+    case 6: launch_stencil_ns<T, 6>(c, dmask, react); break;
+    case 7: launch_stencil_ns<T, 7>(c, dmask, react); break;
+    default: launch_stencil_ns<T, 8>(c, dmask, react); break;
+  }
+}
+
+/* One step: gate (if due), stencil+reaction, fluxes; then swap. */
+template <typename T>
+int step_one(cs_context* c, uint64_t ti, const double* d_uni, uint64_t uni_base, hipEvent_t e0,
+             hipEvent_t e1)
+{
+  const cs_problem& p = c->prob;
+  uint32_t dmask = 0;
+  for (uint32_t s = 0; s < c->ns; s++)
+    if (c->substeps[s] && ti % c->substeps[s] == 0) dmask |= 1u << s;
+  const uint32_t react = (p.num_buffers > 0 && ti % p.reaction_substep == 0) ? 1u : 0u;
+  uint32_t fluxmask = 0;
+  for (uint32_t f = 0; f < c->nf; f++)
+    if (ti % c->flux_substep[f] == 0) fluxmask |= 1u << f;
+  const T* ucur = (const T*)c->d_u[c->cur];
+  if (c->ndiscrete && ti % p.stochastic_substep == 0) {
+    hipLaunchKernelGGL(k_gate<T>, dim3(1), dim3(GATE_THREADS), 0, c->stream, ucur, c->d_pairs,
+                       c->d_states, (const DevParams<T>*)c->d_params, ti, (const uint64_t*)nullptr,
+                       p.stochastic_substep, (uint32_t)p.seed, (uint32_t)p.instance, d_uni, uni_base);
+  }
+  if (e0) HIP_TRY(hipEventRecord(e0, c->stream));
+  launch_stencil<T>(c, dmask, react);
+  if (e1) HIP_TRY(hipEventRecord(e1, c->stream));
+  if (c->nrows && fluxmask) {
+    const uint32_t blocks = (c->nrows + FLUX_THREADS - 1) / FLUX_THREADS;
+    hipLaunchKernelGGL(k_flux<T>, dim3(blocks), dim3(FLUX_THREADS), 0, c->stream, ucur,
+                       (T*)c->d_u[c->cur ^ 1], c->d_ids, c->d_row_voxel, c->d_row_species,
+                       c->d_row_start, c->d_ent_pair, c->d_ent_flux, c->d_ent_sign, c->d_pairs,
+                       c->d_states, (const DevParams<T>*)c->d_params, c->geo, c->nrows, fluxmask);
+  }
+  HIP_TRY(hipGetLastError());
+  c->cur ^= 1;
+  return CS_OK;
+}
+
+uint64_t gates_in(const cs_context* c, uint64_t first, uint64_t n)
+{
+  if (!c->ndiscrete) return 0;
+  const uint64_t D = c->prob.stochastic_substep;
+  /* multiples of D in [first, first+n) */
+  const uint64_t last = first + n;
+  auto upto = [D](uint64_t k) { return k == 0 ? 0 : (k - 1) / D + 1; }; /* multiples in [0,k) incl 0 */
+  return upto(last) - upto(first);
+}
+
+template <typename T>
+int run_steps(cs_context* c, uint64_t first, uint64_t n, const double* host_uni, uint64_t count,
+              bool timing, double* total_ms, double* kms)
+{
+  HIP_TRY(hipSetDevice(c->device));
+  const double* d_uni = nullptr;
+  if (c->prob.rng == CS_RNG_MT && c->ndiscrete && c->stoch_per_gate) {
+    const uint64_t need = gates_in(c, first, n) * c->stoch_per_gate;
+    if (!host_uni) return set_err(CS_EINVAL, "this context draws gating uniforms from the caller: use cs_step_uniforms");
+    if (count != need)
+      return set_err(CS_EINVAL, "expected %llu uniforms, got %llu", (unsigned long long)need,
+                     (unsigned long long)count);
+    if (need > c->uniform_cap) {
+      if (c->d_uniforms) HIP_TRY(hipFree(c->d_uniforms));
+      c->d_uniforms = nullptr;
+      HIP_TRY(hipMalloc(&c->d_uniforms, sizeof(double) * need));
+      c->uniform_cap = need;
+    }
+    if (need)
+      HIP_TRY(hipMemcpyAsync(c->d_uniforms, host_uni, sizeof(double) * need, hipMemcpyHostToDevice,
+                             c->stream));
+    d_uni = c->d_uniforms;
+  } else if (host_uni && count) {
+    return set_err(CS_EINVAL, "this context uses Philox gating; no uniforms expected");
+  }
+  /* timing: one event pair around every stencil launch plus one pair
+   * around the range, all on the context stream, read after the range */
+  std::vector<hipEvent_t> evs;
+  struct EvGuard {
+    std::vector<hipEvent_t>& v;
+    ~EvGuard() { for (auto e : v) if (e) hipEventDestroy(e); }
+  } guard{evs};
+  if (timing) {
+    if (n > 100000) return set_err(CS_ERANGE, "timing ranges are limited to 100000 steps");
+    evs.assign(2 * n + 2, nullptr);
+    for (auto& e : evs) HIP_TRY(hipEventCreate(&e));
+    HIP_TRY(hipEventRecord(evs[2 * n], c->stream));
+  }
+  uint64_t gate_idx = 0;
+  for (uint64_t i = 0; i < n; i++) {
+    const uint64_t ti = first + i;
+    const bool gate = c->ndiscrete && ti % c->prob.stochastic_substep == 0;
+    int rc = step_one<T>(c, ti, d_uni, gate_idx * c->stoch_per_gate,
+                         timing ? evs[2 * i] : nullptr, timing ? evs[2 * i + 1] : nullptr);
+    if (rc != CS_OK) return rc;
+    if (gate) gate_idx++;
+  }
+  if (timing) {
+    HIP_TRY(hipEventRecord(evs[2 * n + 1], c->stream));
+    HIP_TRY(hipEventSynchronize(evs[2 * n + 1]));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, evs[2 * n], evs[2 * n + 1]));
+    *total_ms = ms;
+    double ksum = 0.0;
+    for (uint64_t i = 0; i < n; i++) {
+      HIP_TRY(hipEventElapsedTime(&ms, evs[2 * i], evs[2 * i + 1]));
+      ksum += ms;
+    }
+    *kms = n ? ksum / (double)n : 0.0;
+  }
+  return CS_OK;
+}
+
+int build_flux_rows(cs_context* c, const cs_problem* p)
+{
+  struct Ent { uint64_t key; uint32_t pair; uint8_t flux; int8_t sign; };
+  std::vector<Ent> ents;
+  std::vector<uint32_t> pairs_dev;
+  for (uint32_t f = 0; f < p->num_fluxes; f++) {
+    const cs_flux_desc& F = p->fluxes[f];
+    for (uint32_t i = 0; i < F.num_pairs; i++) {
+      const uint32_t from = ref_to_dev(F.pairs[2 * i], c->n);
+      const uint32_t to = ref_to_dev(F.pairs[2 * i + 1], c->n);
+      pairs_dev.push_back(from);
+      pairs_dev.push_back(to);
+      const uint64_t sp = F.species;
+      ents.push_back({(sp << 32) | to, i, (uint8_t)f, (int8_t)1});
+      if (F.inner) ents.push_back({(sp << 32) | from, i, (uint8_t)f, (int8_t)-1});
+    }
+  }
+  std::stable_sort(ents.begin(), ents.end(), [](const Ent& a, const Ent& b) { return a.key < b.key; });
+  std::vector<uint32_t> row_voxel, row_start, ent_pair;
+  std::vector<uint8_t> row_species, ent_flux;
+  std::vector<int8_t> ent_sign;
+  for (size_t i = 0; i < ents.size(); i++) {
+    if (i == 0 || ents[i].key != ents[i - 1].key) {
+      row_voxel.push_back((uint32_t)(ents[i].key & 0xFFFFFFFFu));
+      row_species.push_back((uint8_t)(ents[i].key >> 32));
+      row_start.push_back((uint32_t)i);
+    }
+    ent_pair.push_back(ents[i].pair);
+    ent_flux.push_back(ents[i].flux);
+    ent_sign.push_back(ents[i].sign);
+  }
+  row_start.push_back((uint32_t)ents.size());
+  c->nrows = (uint32_t)row_voxel.size();
+  auto up = [c](auto** dptr, const auto& vec) -> int {
+    using E = typename std::decay<decltype(vec)>::type::value_type;
+    const size_t bytes = sizeof(E) * std::max<size_t>(vec.size(), 1);
+    HIP_TRY(hipMalloc((void**)dptr, bytes));
+    if (!vec.empty())
+      HIP_TRY(hipMemcpyAsync(*dptr, vec.data(), sizeof(E) * vec.size(), hipMemcpyHostToDevice, c->stream));
+    return CS_OK;
+  };
+  int rc;
+  if ((rc = up(&c->d_pairs, pairs_dev)) != CS_OK) return rc;
+  if ((rc = up(&c->d_row_voxel, row_voxel)) != CS_OK) return rc;
+  if ((rc = up(&c->d_row_species, row_species)) != CS_OK) return rc;
+  if ((rc = up(&c->d_row_start, row_start)) != CS_OK) return rc;
+  if ((rc = up(&c->d_ent_pair, ent_pair)) != CS_OK) return rc;
+  if ((rc = up(&c->d_ent_flux, ent_flux)) != CS_OK) return rc;
+  if ((rc = up(&c->d_ent_sign, ent_sign)) != CS_OK) return rc;
+  /* channel states */
+  c->state_off.assign(p->num_fluxes, 0);
+  c->npairs.assign(p->num_fluxes, 0);
+  c->init_states.clear();
+  c->ndiscrete = 0;
+  c->stoch_per_gate = 0;
+  for (uint32_t f = 0; f < p->num_fluxes; f++) {
+    const cs_flux_desc& F = p->fluxes[f];
+    c->npairs[f] = F.num_pairs;
+    c->flux_substep[f] = F.substep;
+    if (!F.discrete) continue;
+    c->ndiscrete++;
+    c->state_off[f] = (uint32_t)c->init_states.size();
+    for (uint32_t i = 0; i < F.num_pairs; i++) c->init_states.push_back(F.init_states[i] ? 1 : 0);
+    if (!(F.params[8] > 0)) c->stoch_per_gate += F.num_pairs;
+  }
+  if ((rc = up(&c->d_states, c->init_states)) != CS_OK) return rc;
+  return CS_OK;
+}
+
+}  // namespace
+
+/* ------------------------------------------------------------- C ABI */
+
+extern "C" {
+
+const char* cs_last_error(void) { return g_err.c_str(); }
+
+int cs_device_info(int* device, char* name, size_t name_len)
+{
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, dev));
+  if (device) *device = dev;
+  if (name && name_len) {
+    snprintf(name, name_len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+  }
+  return CS_OK;
+}
+
+int cs_create(const cs_problem* p, cs_context** out)
+{
+  if (!out) return set_err(CS_EINVAL, "out is NULL");
+  *out = nullptr;
+  int rc = validate(p);
+  if (rc != CS_OK) return rc;
+  cs_context* c = new cs_context();
+  c->prob = *p;
+  c->prob.domain_ids = nullptr;
+  for (auto& F : c->prob.fluxes) { F.pairs = nullptr; F.init_states = nullptr; }
+  c->precision = p->precision;
+  c->esize = p->precision == CS_PREC_F64 ? 8 : 4;
+  c->ns = p->num_species;
+  c->nd = p->num_domains;
+  c->nf = p->num_fluxes;
+  for (int a = 0; a < 3; a++) c->n[a] = p->n[a];
+  for (uint32_t s = 0; s < c->ns; s++) c->substeps[s] = p->substeps[s];
+  Geo& g = c->geo;
+  g.nx = (int)p->n[0];
+  g.ny = (int)p->n[1];
+  g.nz = (int)p->n[2];
+  g.plane = (uint64_t)g.nx * g.ny;
+  g.nv = g.plane * g.nz;
+  auto bail = [&](int code) { free_ctx(c); return code; };
+  if (hipGetDevice(&c->device) != hipSuccess) return bail(set_err(CS_EDEVICE, "no HIP device"));
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(set_err(CS_EDEVICE, "hipStreamCreate failed"));
+
+  /* domain ids in device order with the uniform-neighbourhood bit */
+  std::vector<uint8_t> ids(g.nv);
+  const uint32_t n0 = p->n[0], n1 = p->n[1], n2 = p->n[2];
+  for (uint32_t x = 0; x < n0; x++)
+    for (uint32_t y = 0; y < n1; y++)
+      for (uint32_t z = 0; z < n2; z++) {
+        const uint64_t r = ((uint64_t)x * n1 + y) * n2 + z;
+        const uint8_t d = p->domain_ids[r];
+        bool uni = true;
+        if (x > 0) uni &= p->domain_ids[r - (uint64_t)n1 * n2] == d;
+        if (x + 1 < n0) uni &= p->domain_ids[r + (uint64_t)n1 * n2] == d;
+        if (y > 0) uni &= p->domain_ids[r - n2] == d;
+        if (y + 1 < n1) uni &= p->domain_ids[r + n2] == d;
+        if (z > 0) uni &= p->domain_ids[r - 1] == d;
+        if (z + 1 < n2) uni &= p->domain_ids[r + 1] == d;
+        ids[((uint64_t)z * n1 + y) * n0 + x] = (uint8_t)(d | (uni ? 0x80 : 0));
+      }
+  if (hipMalloc(&c->d_ids, g.nv) != hipSuccess) return bail(set_err(CS_ENOMEM, "device allocation of ids failed"));
+  for (int b = 0; b < 2; b++)
+    if (hipMalloc(&c->d_u[b], c->esize * c->ns * g.nv) != hipSuccess)
+      return bail(set_err(CS_ENOMEM, "device allocation of %zu bytes failed", c->esize * c->ns * g.nv));
+  if (hipMemcpyAsync(c->d_ids, ids.data(), g.nv, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+    return bail(set_err(CS_EDEVICE, "upload of ids failed"));
+  rc = build_flux_rows(c, p);
+  if (rc != CS_OK) return bail(rc);
+  rc = c->esize == 8 ? upload_params<double>(c) : upload_params<float>(c);
+  if (rc != CS_OK) return bail(rc);
+  if (hipMalloc(&c->d_partial, sizeof(double) * STAT_BLOCKS * MAXS * MAXD * 3) != hipSuccess ||
+      hipMalloc(&c->d_stats, sizeof(double) * MAXS * MAXD * 3) != hipSuccess)
+    return bail(set_err(CS_ENOMEM, "device allocation of reduction buffers failed"));
+  /* z split: enough blocks to cover the 256 CUs several times over */
+  const uint64_t xy_blocks = (uint64_t)((g.nx + BX - 1) / BX) * ((g.ny + BY - 1) / BY);
+  int zs = (int)std::max<uint64_t>(1, (2048 + xy_blocks - 1) / xy_blocks);
+  zs = std::min(zs, std::max(1, g.nz / 8));
+  zs = std::max(1, std::min(zs, g.nz));
+  c->zchunk = (g.nz + zs - 1) / zs;
+  c->zsplit = (g.nz + c->zchunk - 1) / c->zchunk;
+  rc = c->esize == 8 ? fill_init<double>(c) : fill_init<float>(c);
+  if (rc != CS_OK) return bail(rc);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(set_err(CS_EDEVICE, "setup failed"));
+  *out = c;
+  return CS_OK;
+}
+
+int cs_destroy(cs_context* c)
+{
+  if (!c) return CS_OK;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  free_ctx(c);
+  return CS_OK;
+}
+
+int cs_reset(cs_context* c)
+{
+  if (!c) return set_err(CS_EINVAL, "context is NULL");
+  HIP_TRY(hipSetDevice(c->device));
+  int rc = c->esize == 8 ? fill_init<double>(c) : fill_init<float>(c);
+  if (rc != CS_OK) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return CS_OK;
+}
+
+int cs_upload_species(cs_context* c, uint32_t s, const double* src)
+{
+  if (!c || !src) return set_err(CS_EINVAL, "NULL argument");
+  if (s >= c->ns) return set_err(CS_ERANGE, "species %u out of range", s);
+  HIP_TRY(hipSetDevice(c->device));
+  const uint64_t nv = c->geo.nv;
+  std::vector<char> buf(nv * c->esize);
+  const uint32_t n0 = c->n[0], n1 = c->n[1], n2 = c->n[2];
+  for (uint32_t x = 0; x < n0; x++)
+    for (uint32_t y = 0; y < n1; y++)
+      for (uint32_t z = 0; z < n2; z++) {
+        const double val = src[((uint64_t)x * n1 + y) * n2 + z];
+        const uint64_t dv = ((uint64_t)z * n1 + y) * n0 + x;
+        if (c->esize == 8) ((double*)buf.data())[dv] = val;
+        else ((float*)buf.data())[dv] = (float)val;
+      }
+  char* base = (char*)c->d_u[c->cur] + (uint64_t)s * nv * c->esize;
+  HIP_TRY(hipMemcpyAsync(base, buf.data(), buf.size(), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return CS_OK;
+}
+
+int cs_download_species(cs_context* c, uint32_t s, double* dst)
+{
+  if (!c || !dst) return set_err(CS_EINVAL, "NULL argument");
+  if (s >= c->ns) return set_err(CS_ERANGE, "species %u out of range", s);
+  HIP_TRY(hipSetDevice(c->device));
+  const uint64_t nv = c->geo.nv;
+  std::vector<char> buf(nv * c->esize);
+  const char* base = (const char*)c->d_u[c->cur] + (uint64_t)s * nv * c->esize;
+  HIP_TRY(hipMemcpyAsync(buf.data(), base, buf.size(), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  const uint32_t n0 = c->n[0], n1 = c->n[1], n2 = c->n[2];
+  for (uint32_t x = 0; x < n0; x++)
+    for (uint32_t y = 0; y < n1; y++)
+      for (uint32_t z = 0; z < n2; z++) {
+        const uint64_t dv = ((uint64_t)z * n1 + y) * n0 + x;
+        dst[((uint64_t)x * n1 + y) * n2 + z] =
+            c->esize == 8 ? ((const double*)buf.data())[dv] : (double)((const float*)buf.data())[dv];
+      }
+  return CS_OK;
+}
+
+int cs_uniforms_needed(const cs_context* c, uint64_t first, uint64_t n, uint64_t* count)
+{
+  if (!c || !count) return set_err(CS_EINVAL, "NULL argument");
+  *count = c->prob.rng == CS_RNG_MT ? gates_in(c, first, n) * c->stoch_per_gate : 0;
+  return CS_OK;
+}
+
+int cs_step(cs_context* c, uint64_t first, uint64_t n)
+{
+  if (!c) return set_err(CS_EINVAL, "context is NULL");
+  if (first == 0) return set_err(CS_EINVAL, "time indices start at 1");
+  return c->esize == 8 ? run_steps<double>(c, first, n, nullptr, 0, false, nullptr, nullptr)
+                       : run_steps<float>(c, first, n, nullptr, 0, false, nullptr, nullptr);
+}
+
+int cs_step_uniforms(cs_context* c, uint64_t first, uint64_t n, const double* uni, uint64_t count)
+{
+  if (!c) return set_err(CS_EINVAL, "context is NULL");
+  if (first == 0) return set_err(CS_EINVAL, "time indices start at 1");
+  if (count && !uni) return set_err(CS_EINVAL, "uniforms is NULL");
+  static const double dummy = 0.0;
+  if (!uni) uni = &dummy;
+  return c->esize == 8 ? run_steps<double>(c, first, n, uni, count, false, nullptr, nullptr)
+                       : run_steps<float>(c, first, n, uni, count, false, nullptr, nullptr);
+}
+
+int cs_synchronize(cs_context* c)
+{
+  if (!c) return set_err(CS_EINVAL, "context is NULL");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return CS_OK;
+}
+
+int cs_time_steps(cs_context* c, uint64_t first, uint64_t n, double* total_ms, double* kernel_ms)
+{
+  if (!c || !total_ms || !kernel_ms) return set_err(CS_EINVAL, "NULL argument");
+  if (first == 0) return set_err(CS_EINVAL, "time indices start at 1");
+  if (c->prob.rng == CS_RNG_MT && c->stoch_per_gate)
+    return set_err(CS_EINVAL, "cs_time_steps needs Philox gating");
+  return c->esize == 8 ? run_steps<double>(c, first, n, nullptr, 0, true, total_ms, kernel_ms)
+                       : run_steps<float>(c, first, n, nullptr, 0, true, total_ms, kernel_ms);
+}
+
+int cs_reduce_stats(cs_context* c, double* mn, double* mx, double* tot)
+{
+  if (!c || !mn || !mx || !tot) return set_err(CS_EINVAL, "NULL argument");
+  HIP_TRY(hipSetDevice(c->device));
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>(STAT_BLOCKS, (c->geo.nv + STAT_THREADS - 1) / STAT_THREADS);
+  const double dV = c->prob.dV;
+  if (c->esize == 8)
+    hipLaunchKernelGGL(k_stats<double>, dim3(blocks), dim3(STAT_THREADS), 0, c->stream,
+                       (const double*)c->d_u[c->cur], c->d_ids, c->geo, c->ns, dV, c->d_partial);
+  else
+    hipLaunchKernelGGL(k_stats<float>, dim3(blocks), dim3(STAT_THREADS), 0, c->stream,
+                       (const float*)c->d_u[c->cur], c->d_ids, c->geo, c->ns, dV, c->d_partial);
+  hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(MAXS * MAXD), 0, c->stream, c->d_partial, blocks,
+                     c->ns, c->d_stats);
+  HIP_TRY(hipGetLastError());
+  double host[MAXS * MAXD * 3];
+  HIP_TRY(hipMemcpyAsync(host, c->d_stats, sizeof(double) * c->ns * MAXD * 3, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (uint32_t d = 0; d < c->nd; d++)
+    for (uint32_t s = 0; s < c->ns; s++) {
+      const double* h = host + (s * MAXD + d) * 3;
+      mn[d * c->ns + s] = h[0];
+      mx[d * c->ns + s] = h[1];
+      tot[d * c->ns + s] = h[2];
+    }
+  return CS_OK;
+}
+
+int cs_get_channel_states(cs_context* c, uint32_t f, uint8_t* states)
+{
+  if (!c || !states) return set_err(CS_EINVAL, "NULL argument");
+  if (f >= c->nf || !c->prob.fluxes[f].discrete) return set_err(CS_ERANGE, "flux %u is not a discrete flux", f);
+  HIP_TRY(hipSetDevice(c->device));
+  if (c->npairs[f])
+    HIP_TRY(hipMemcpyAsync(states, c->d_states + c->state_off[f], c->npairs[f], hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return CS_OK;
+}
+
+int cs_set_channel_states(cs_context* c, uint32_t f, const uint8_t* states)
+{
+  if (!c || !states) return set_err(CS_EINVAL, "NULL argument");
+  if (f >= c->nf || !c->prob.fluxes[f].discrete) return set_err(CS_ERANGE, "flux %u is not a discrete flux", f);
+  HIP_TRY(hipSetDevice(c->device));
+  std::vector<uint8_t> tmp(states, states + c->npairs[f]);
+  for (auto& v : tmp) v = v ? 1 : 0;
+  if (c->npairs[f])
+    HIP_TRY(hipMemcpyAsync(c->d_states + c->state_off[f], tmp.data(), c->npairs[f], hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return CS_OK;
+}
+
+double cs_ryr_flux(double dt, double h, double u0, double u1, const double* params)
+{
+  return cs_ryr_flux_d(dt, h, u0, u1, params[6]);
+}
+
+double cs_serca_flux(double dt, double h, double u0, double u1, const double* params)
+{
+  return cs_serca_flux_d(dt, h, u0, u1, params[1]);
+}
+
+void cs_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+  uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+  cs_philox(c, key[0], key[1]);
+  for (int i = 0; i < 4; i++) out[i] = c[i];
+}
+
+}  // extern "C"

@@ -1,0 +1,173 @@
+/*
+ * csparks.h -- C ABI of the MI355X time-stepping library (libcsparks.so).
+ *
+ * One call of cs_step() advances the resident domain by whole time steps of
+ * the reference's explicit scheme (calcium_sparks.c:190-228):
+ *
+ *   diffusion      Species_step_diffusion          species.c:1501-1650
+ *                  kernel_SIMD / kernel_LOTR       kernel_generic.cpp:512-600
+ *                  side_SIMD + scalar borders      kernel_generic.cpp:175-229,
+ *                                                  species.c:1282-1498
+ *   gating         Species_evaluate_stochastic_events species.c:1910-1957
+ *                  evaluate_RyR_stochastically     boundaryfluxes.c:295-359
+ *   boundary flux  Species_step_boundary_fluxes    species.c:1704-1807
+ *                  ryr_flux / serca_flux           boundaryfluxes.c:87-141, :12-85
+ *   reaction       Species_step_reaction           species.c:1653-1702
+ *                  reaction_SIMD                   kernel_generic.cpp:397-509
+ *   apply du       Species_apply_du                species.c:1080-1121
+ *   fixed values   Species_apply_fixed_values      species.c:1037-1077
+ *
+ * The whole state lives on one GPU; there is no ghost exchange.  Host
+ * arrays crossing this boundary use the reference layout [n0][n1][n2]
+ * (z fastest) and double precision, whatever precision the device uses.
+ *
+ * Errors: every entry point returns CS_OK (0) or a negative CS_E* code and
+ * records a message retrievable with cs_last_error().  Contexts are not
+ * re-entrant: one host thread per context.
+ */
+#ifndef CSPARKS_H
+#define CSPARKS_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "cs_model.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CS_MAX_FLUXES 4
+#define CS_NUM_RYR_PARAMS 10
+
+enum {
+  CS_OK = 0,
+  CS_EINVAL = -1,     /* bad argument / inconsistent descriptor          */
+  CS_EDEVICE = -2,    /* HIP runtime error                               */
+  CS_ENOMEM = -3,     /* device or host allocation failed                */
+  CS_ERANGE = -4      /* index or size out of range                      */
+};
+
+/* Flux models of the reference's plugin table (basefluxinfo.h:34-46). */
+enum { CS_FLUX_RYR = 0, CS_FLUX_SERCA = 1 };
+
+/* One boundary flux.  pairs[] holds (from, to) linear voxel indices
+ * x*n1*n2 + y*n2 + z of the refined grid (reference species.c:938-1035).
+ * params[] keeps the reference's parameter vector order:
+ *   RyR  : Kd_close, k_max_close, n_open, k_max_open, k_min_open, n_close,
+ *          i_unitary, k_min_close, t_close, Kd_open (boundaryfluxes.c:447-456)
+ *   SERCA: v_cyt_to_A_sr, density, scale            (boundaryfluxes.c:536-538)
+ * Discrete fluxes are channels gated every stochastic substep; a positive
+ * t_close (params[8]) closes them deterministically at t_close without
+ * drawing random numbers. */
+typedef struct cs_flux_desc {
+  uint32_t kind;
+  uint32_t species;          /* species slot moved across the boundary   */
+  uint32_t substep;          /* fires when time_ind % substep == 0       */
+  uint32_t inner;            /* 1: also subtract J at the from voxel     */
+  uint32_t discrete;         /* 1: channel pairs with open/closed state  */
+  uint32_t num_pairs;
+  const uint32_t* pairs;     /* [num_pairs][2]                           */
+  double   params[CS_NUM_RYR_PARAMS];
+  const uint8_t* init_states;/* [num_pairs] 0/1, discrete only           */
+} cs_flux_desc;
+
+/* Everything one time step needs, already derived from the input files
+ * (dt, substeps, coefficient tables, refined ids, boundary voxels). */
+typedef struct cs_problem {
+  uint32_t n[3];             /* refined voxels per axis                  */
+  uint32_t num_domains, num_species;
+  uint32_t precision;        /* CS_PREC_F64 / CS_PREC_F32                */
+  uint32_t rng;              /* CS_RNG_PHILOX / CS_RNG_MT                */
+  uint64_t seed;             /* Philox key word 0                        */
+  uint64_t instance;         /* Philox key word 1                        */
+  double   h, dt, dV;
+  const uint8_t* domain_ids; /* [n0][n1][n2], z fastest                  */
+  double   init[CS_MAX_DOMAINS][CS_MAX_SPECIES];
+  /* diffusion: substeps[s] == 0 marks a non-diffusive species; coef[s] is
+   * the 5x5 table dt*k_s*sigma/h^2 of species.c:1547-1560 */
+  uint32_t substeps[CS_MAX_SPECIES];
+  double   coef[CS_MAX_SPECIES][CS_MAX_DOMAINS * CS_MAX_DOMAINS];
+  /* reaction tables, transposed [buffer][domain] like species.c:249-253 */
+  uint32_t num_buffers, reaction_substep;
+  double   dt_react;
+  uint8_t  bsp0[CS_MAX_SPECIES][CS_MAX_DOMAINS];
+  uint8_t  bsp1[CS_MAX_SPECIES][CS_MAX_DOMAINS];
+  double   k_on[CS_MAX_SPECIES][CS_MAX_DOMAINS];
+  double   k_off[CS_MAX_SPECIES][CS_MAX_DOMAINS];
+  double   tot[CS_MAX_SPECIES][CS_MAX_DOMAINS];
+  uint32_t num_fluxes;
+  cs_flux_desc fluxes[CS_MAX_FLUXES];  /* discrete ones in gating order  */
+  uint32_t stochastic_substep;
+  uint32_t force_dirichlet;  /* -f: species 0 on the outer faces := init */
+  uint8_t  fixed[CS_MAX_DOMAINS][CS_MAX_SPECIES]; /* fixed domain species */
+} cs_problem;
+
+typedef struct cs_context cs_context;
+
+/* Replaces Species_construct + Species_init_values + Species_init_du
+ * (species.c:95-884, :887-935, :22-32): allocates the device state, uploads
+ * the problem and fills every voxel with init[domain][species]. */
+int cs_create(const cs_problem* problem, cs_context** ctx);
+int cs_destroy(cs_context* ctx);
+
+/* Device ordinal the library runs on, and its name. */
+int cs_device_info(int* device, char* name, size_t name_len);
+
+/* Reset the fields to init[domain][species] and the channels to the
+ * problem's init_states (Species_init_values, species.c:887-935). */
+int cs_reset(cs_context* ctx);
+
+/* Field transfer in the reference layout, double precision. */
+int cs_upload_species(cs_context* ctx, uint32_t species, const double* src);
+int cs_download_species(cs_context* ctx, uint32_t species, double* dst);
+
+/* Advance nsteps steps with time indices first_time_ind .. +nsteps-1
+ * (the reference's time_ind starts at 1).  Asynchronous on the context's
+ * stream.  In CS_RNG_MT mode use cs_step_uniforms instead. */
+int cs_step(cs_context* ctx, uint64_t first_time_ind, uint64_t nsteps);
+
+/* Same as cs_step, with the gating uniforms supplied by the caller: for
+ * every gating step in the range (time_ind % stochastic_substep == 0), in
+ * order, one draw per channel of every discrete flux whose t_close <= 0,
+ * fluxes in descriptor order.  count must equal that total. */
+int cs_step_uniforms(cs_context* ctx, uint64_t first_time_ind, uint64_t nsteps,
+                     const double* uniforms, uint64_t count);
+
+/* Number of uniforms cs_step_uniforms needs for a step range. */
+int cs_uniforms_needed(const cs_context* ctx, uint64_t first_time_ind,
+                       uint64_t nsteps, uint64_t* count);
+
+int cs_synchronize(cs_context* ctx);
+
+/* Per (domain, species) min, max and sum(u*dV) of the current fields,
+ * arrays of num_domains*num_species, index d*num_species+s.  Max starts
+ * from FLT_MIN and min from FLT_MAX like species.c:2808-2814; sums are
+ * accumulated in double (Species_output_scalar_data, species.c:2785-2863). */
+int cs_reduce_stats(cs_context* ctx, double* min_values, double* max_values,
+                    double* total_values);
+
+/* Channel states of discrete flux f (0 closed, 1 open). */
+int cs_get_channel_states(cs_context* ctx, uint32_t flux, uint8_t* states);
+int cs_set_channel_states(cs_context* ctx, uint32_t flux, const uint8_t* states);
+
+/* Time nsteps steps on the context stream with HIP events: total wall
+ * time of the range and the mean duration of the fused stencil/reaction
+ * kernel launches (ms).  Uses Philox gating. */
+int cs_time_steps(cs_context* ctx, uint64_t first_time_ind, uint64_t nsteps,
+                  double* total_ms, double* stencil_kernel_ms);
+
+/* Scalar flux functions of the plugin table, evaluated on the host with
+ * the same code the device runs (ryr_flux boundaryfluxes.c:87-141,
+ * serca_flux :12-85). */
+double cs_ryr_flux(double dt, double h, double u0, double u1, const double* params);
+double cs_serca_flux(double dt, double h, double u0, double u1, const double* params);
+
+/* Philox4x32-10 block (Salmon et al., SC'11), exposed for known-answer tests. */
+void cs_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+const char* cs_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CSPARKS_H */

@@ -1,0 +1,44 @@
+"""Shared test inputs: small dyads the oracle steps in seconds."""
+from __future__ import annotations
+
+import numpy as np
+
+import csparks as cs
+import dyad
+
+
+def with_serca(d: dyad.Dyad, density: float = 75.0) -> dyad.Dyad:
+    """Add a network-SR slab under the top face and a SERCA density
+    boundary from the cytosol layer below it into the SR (the reference's
+    'serca' boundary of the shipped geometries, boundaryfluxes.c:12-85)."""
+    g = d.geometry
+    vox = g.voxels.copy()
+    nx, ny, nz = vox.shape
+    vox[2:nx - 2, 2:ny - 2, nz - 2:] = dyad.NSR
+    pairs = []
+    for x in range(2, nx - 2):
+        for y in range(2, ny - 2):
+            pairs.append((x, y, nz - 3, x, y, nz - 2))
+    bnds = list(g.boundaries) + [cs.Boundary("serca", "density", np.array(pairs, dtype=np.uint16))]
+    geom = cs.Geometry(voxels=vox, h=g.h, domain_names=list(g.domain_names),
+                       domain_connections=g.domain_connections, boundaries=bnds)
+    m = d.model
+    model = cs.SpeciesModel(domain_names=m.domain_names, species_names=m.species_names,
+                            diffusive=m.diffusive, sigma=m.sigma, init=m.init, buffers=m.buffers,
+                            ryr=m.ryr, p_ryr=m.p_ryr,
+                            serca=cs.SercaModel(307.0, density, 1.0, "serca", "Ca"))
+    return dyad.Dyad(geom, model, d.open_index, d.description + " + serca")
+
+
+def rel_err(a: np.ndarray, b: np.ndarray) -> float:
+    scale = max(float(np.max(np.abs(b))), 1e-300)
+    return float(np.max(np.abs(a - b)) / scale)
+
+
+def species_close(a: np.ndarray, b: np.ndarray, rtol: float):
+    """Per-voxel relative error, with an absolute floor of rtol times the
+    field's magnitude for voxels whose value is (near) zero."""
+    scale = float(np.max(np.abs(b)))
+    tol = rtol * np.maximum(np.abs(b), 1e-6 * scale)
+    bad = np.abs(a - b) > tol
+    return not bad.any(), float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-6 * max(scale, 1e-300))))

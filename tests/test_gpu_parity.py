@@ -1,0 +1,182 @@
+"""GPU parity: the HIP path against the CPU oracle on the same inputs.
+
+Deterministic FP64 runs must agree to 1e-10 relative per voxel (the bar of
+a deterministic -O/-C run); stochastic runs draw the same Philox (or
+Mersenne Twister) uniforms on both sides, so channel trajectories must be
+identical and fields agree like deterministic runs; FP32 fields are held
+to a stated per-species bound against the FP64 oracle.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import csparks as cs
+import dyad
+from helpers import species_close, with_serca
+from oracle_py import Oracle
+
+pytestmark = pytest.mark.gpu
+
+RTOL_F64 = 1e-10
+RTOL_F32 = 2e-4
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _compare_fields(sim, orc, rtol):
+    worst = 0.0
+    for s in range(orc.ns):
+        ok, err = species_close(sim.species(s), orc.species(s), rtol)
+        worst = max(worst, err)
+        assert ok, f"species {s}: max rel err {err:.3e} > {rtol}"
+    return worst
+
+
+def _run_pair(d, opts, nsteps, chunks=1):
+    _say(f"case: {d.description} opts={opts} steps={nsteps}")
+    orc = Oracle(d.geometry, d.model, opts)
+    sim = cs.Simulation(d.geometry, d.model, opts)
+    per = nsteps // chunks
+    for _ in range(chunks):
+        sim.step(per)
+        orc.step(per)
+    sim.synchronize()
+    return sim, orc
+
+
+def test_device_visible():
+    _say(cs.device_info())
+
+
+def test_fp64_deterministic_small():
+    d = dyad.small_dyad()
+    opts = cs.Options(seed=7, open={"ryr": [d.open_index, 4]}, t_close={"ryr": 1.0}, tstop=1.0)
+    sim, orc = _run_pair(d, opts, 300, chunks=3)
+    worst = _compare_fields(sim, orc, RTOL_F64)
+    _say(f"fp64 deterministic max rel err {worst:.2e}")
+    np.testing.assert_array_equal(sim.channel_states(0), orc.states(0))
+    sim.close()
+
+
+def test_fp64_forced_close_and_substeps():
+    """-C closes the forced-open channel mid-run; -T/-D substeps > 1."""
+    d = dyad.small_dyad(seed=5)
+    opts = cs.Options(seed=3, open={"ryr": [d.open_index]}, t_close={"ryr": 0.002, "p_ryr": 0.001},
+                      dt_update_react=3.5e-5 * 3, dt_update_stoch=3.5e-5 * 2)
+    sim, orc = _run_pair(d, opts, 240, chunks=2)
+    assert sim.problem.reaction_substep > 1 and sim.problem.stochastic_substep > 1
+    _compare_fields(sim, orc, RTOL_F64)
+    assert sim.channel_states(0).sum() == 0
+    sim.close()
+
+
+def test_fp64_refined_geometry():
+    """12 nm geometry run at -h 6: two subdivisions, discrete channels on
+    the centre sub-voxel pair (species.c:1003-1020)."""
+    d = dyad.small_dyad(n=(10, 9, 7), h=12.0, seed=11)
+    opts = cs.Options(h=6.0, seed=1, open={"ryr": [0, 1]}, t_close={"ryr": 1.0})
+    sim, orc = _run_pair(d, opts, 150)
+    assert sim.n == (20, 18, 14)
+    _compare_fields(sim, orc, RTOL_F64)
+    sim.close()
+
+
+def test_fp64_serca_and_dirichlet():
+    d = with_serca(dyad.small_dyad(seed=9))
+    opts = cs.Options(seed=2, open={"ryr": [d.open_index]}, t_close={"ryr": 1.0}, force_dirichlet=True)
+    sim, orc = _run_pair(d, opts, 200, chunks=2)
+    _compare_fields(sim, orc, RTOL_F64)
+    sim.close()
+
+
+@pytest.mark.parametrize("rng", ["philox", "mt"])
+def test_stochastic_trajectories_match(rng):
+    """Same uniforms on both sides => identical channel trajectories."""
+    import dataclasses
+    d = dyad.small_dyad(seed=4)
+    # fast gating (rates ~100/ms) makes channels flicker within a few
+    # hundred steps of the stable dt
+    fast = dict(Kd_open=0.01, k_max_open=200.0, k_min_close=150.0, k_max_close=150.0)
+    model = dataclasses.replace(d.model, ryr=dataclasses.replace(d.model.ryr, **fast),
+                                p_ryr=dataclasses.replace(d.model.p_ryr, **fast))
+    opts = cs.Options(seed=123, rng=rng, open={"ryr": [d.open_index]})
+    orc = Oracle(d.geometry, model, opts)
+    sim = cs.Simulation(d.geometry, model, opts)
+    _say(f"case: stochastic {rng}")
+    transitions = 0
+    prev = orc.states(0).copy()
+    for k in range(40):
+        sim.step(5)
+        orc.step(5)
+        for f in range(2):
+            a, b = sim.channel_states(f), orc.states(f)
+            np.testing.assert_array_equal(a, b, err_msg=f"flux {f} after {5 * (k + 1)} steps")
+        transitions += int(np.sum(orc.states(0) != prev))
+        prev = orc.states(0).copy()
+    assert transitions > 0, "no gating events happened; the test exercises nothing"
+    _compare_fields(sim, orc, RTOL_F64)
+    sim.close()
+
+
+def test_fp32_against_fp64_oracle():
+    d = with_serca(dyad.small_dyad(seed=6))
+    opts64 = cs.Options(seed=8, open={"ryr": [d.open_index]}, t_close={"ryr": 1.0})
+    opts32 = cs.Options(seed=8, open={"ryr": [d.open_index]}, t_close={"ryr": 1.0}, precision="f32")
+    orc = Oracle(d.geometry, d.model, opts64)
+    sim = cs.Simulation(d.geometry, d.model, opts32)
+    _say("case: fp32 vs fp64 oracle")
+    sim.step(300)
+    orc.step(300)
+    errs = []
+    for s in range(orc.ns):
+        ok, err = species_close(sim.species(s), orc.species(s), RTOL_F32)
+        errs.append(err)
+        assert ok, f"species {s}: fp32 max rel err {err:.3e} > {RTOL_F32}"
+    _say("fp32 max rel err per species: " + ", ".join(f"{e:.2e}" for e in errs))
+    sim.close()
+
+
+def test_stats_match_oracle():
+    d = dyad.small_dyad(seed=2)
+    opts = cs.Options(seed=5, open={"ryr": [d.open_index]}, t_close={"ryr": 1.0})
+    sim, orc = _run_pair(d, opts, 50)
+    a = sim.stats()
+    b = orc.stats()
+    for x, y in zip(a, b):
+        np.testing.assert_allclose(x, y, rtol=1e-10, atol=0)
+    sim.close()
+
+
+def test_upload_download_roundtrip():
+    d = dyad.small_dyad(seed=1)
+    opts = cs.Options(seed=1)
+    sim = cs.Simulation(d.geometry, d.model, opts)
+    rng = np.random.default_rng(0)
+    for s in range(sim.ns):
+        x = rng.random(sim.n)
+        sim.set_species(s, x)
+        np.testing.assert_array_equal(sim.species(s), x)
+    sim.close()
+
+
+def test_config1_size_conservation():
+    """Full 400x400x200 grid (32M voxels): with no-flux faces and internal
+    fluxes only, total calcium (free + bound, every species slot) is
+    conserved; FP64 must keep it to round-off."""
+    d = dyad.config1(seed=1)
+    opts = cs.Options(h=3.0, seed=1, open={"ryr": list(range(0, 60, 3))}, t_close={"ryr": 5.0})
+    _say("case: config1 conservation fp64")
+    sim = cs.Simulation(d.geometry, d.model, opts)
+    assert sim.n == (400, 400, 200)
+    tot0 = sim.stats()[2].sum()
+    sim.step(50)
+    mn, mx, tot = sim.stats()
+    rel = abs(tot.sum() - tot0) / tot0
+    _say(f"config1 conservation rel drift {rel:.2e}")
+    assert rel < 1e-12
+    assert np.all(mn[:3] > -1e-9)
+    sim.close()
