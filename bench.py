@@ -71,7 +71,17 @@ def main():
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--size", type=int, nargs=3, default=[400, 400, 200])
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--launch", type=str, default="",
+                    help="fixed launch shape rows,zchunk,xcd_remap,prefetch,nontemporal,lds_tile (skips autotune)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy of MIB MiB (for scaling traffic counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        ms = cs.calibration_copy(args.copy_calibration << 20, max(args.steps, 1))
+        print(json.dumps({"copy_bytes": args.copy_calibration << 20, "ms_per_copy": ms,
+                          "GBps": 2 * (args.copy_calibration << 20) / (ms * 1e-3) / 1e9}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "
               "(independent realisations would shard by instance, see DESIGN.md)", file=sys.stderr)
@@ -87,6 +97,15 @@ def main():
 
     sim.step(args.warmup)
     sim.synchronize()
+    launch = sim.launch_config()
+    if args.launch:
+        keys = ["rows", "zchunk", "xcd_remap", "prefetch", "nontemporal", "lds_tile"]
+        sim.set_launch_config(**dict(zip(keys, (int(v) for v in args.launch.split(",")))))
+        launch = sim.launch_config()
+    elif not args.no_autotune:
+        t2 = time.perf_counter()
+        launch, tune_ms = sim.autotune(1)
+        log(f"autotune {time.perf_counter() - t2:.1f}s -> {launch} ({tune_ms * 1e3:.1f} us)")
     t1 = time.perf_counter()
     total_ms, kernel_ms = sim.time_steps(args.steps)
     sim.synchronize()
@@ -139,7 +158,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(kern_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(kern_gbs / PEAK_HBM_GBS, 4), "traffic": traffic,
                      "kernel": "k_stencil", "kernel_ms": round(kernel_ms, 5),
-                     "algorithmic_bytes_per_launch": nv * bpv},
+                     "algorithmic_bytes_per_launch": nv * bpv, "launch": launch},
         "cpu_baseline": cpu,
         "total_calcium": tot,
     }

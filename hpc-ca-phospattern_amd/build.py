@@ -67,6 +67,40 @@ def build_host():
     return out
 
 
+def _hdf5_flags():
+    """Link libhdf5 by path with a RUNPATH that only covers libcsio's own
+    dependencies: a -L/-rpath on the whole prefix would also hand its older
+    libstdc++ to the HIP runtime."""
+    inc = os.path.join(CONDA, "include")
+    lib = os.path.join(CONDA, "lib")
+    so = os.path.join(lib, "libhdf5.so")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")) or not os.path.exists(so):
+        raise RuntimeError(f"HDF5 not found under {CONDA}; set CS_HDF5_PREFIX")
+    return ["-I", inc], [so, "-Wl,--enable-new-dtags", f"-Wl,-rpath,{lib}"]
+
+
+def build_io():
+    os.makedirs(OUT, exist_ok=True)
+    out = os.path.join(OUT, "libcsio.so")
+    cflags, ldflags = _hdf5_flags()
+    _run(["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-Wall", "-I", INC, *cflags,
+          os.path.join(HOST, "cs_io.c"), "-o", out + ".tmp", *ldflags])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_cli():
+    """The calcium_sparks program, linked against the three libraries."""
+    os.makedirs(OUT, exist_ok=True)
+    out = os.path.join(OUT, "calcium_sparks")
+    cflags, _ = _hdf5_flags()
+    _run(["gcc", "-O2", "-std=gnu11", "-Wall", "-I", INC, *cflags, os.path.join(HOST, "calcium_sparks.c"),
+          "-o", out + ".tmp", "-L", OUT, "-lcshost", "-lcsio", "-lcsparks", "-Wl,--enable-new-dtags",
+          "-Wl,-rpath,$ORIGIN", "-lm"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_oracle():
     """The CPU checker (oracle/) and, when the reference tree is present,
     the checker libraries compiled from its self-contained files."""
@@ -79,6 +113,8 @@ def build_oracle():
 def build_all():
     build_host()
     build_device()
+    build_io()
+    build_cli()
     build_oracle()
 
 

@@ -170,6 +170,14 @@ class CSetup(C.Structure):
     ]
 
 
+class CLaunchConfig(C.Structure):
+    _fields_ = [("rows", C.c_uint32), ("zchunk", C.c_uint32), ("xcd_remap", C.c_uint32),
+                ("prefetch", C.c_uint32), ("nontemporal", C.c_uint32), ("lds_tile", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
 class CAbortState(C.Structure):
     _fields_ = [("tabort", C.c_double), ("last_closed", C.c_double),
                 ("all_closed", C.c_int), ("active", C.c_int)]
@@ -249,7 +257,29 @@ def device_lib() -> C.CDLL:
         lib.cs_serca_flux.restype = C.c_double
         lib.cs_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         lib.cs_philox4x32_10.restype = None
+        lib.cs_get_launch_config.argtypes = [P, C.POINTER(CLaunchConfig)]
+        lib.cs_set_launch_config.argtypes = [P, C.POINTER(CLaunchConfig)]
+        lib.cs_autotune.argtypes = [P, C.c_uint64, C.c_uint32, C.POINTER(CLaunchConfig), C.POINTER(C.c_double)]
+        lib.cs_calibration_copy.argtypes = [C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]
         lib.cs_last_error.restype = C.c_char_p
+        lib._cs_typed = True
+    return lib
+
+
+def io_lib() -> C.CDLL:
+    lib = _load("libcsio.so")
+    if not getattr(lib, "_cs_typed", False):
+        lib.cs_io_read_geometry.argtypes = [C.c_char_p, C.POINTER(CGeometry), C.c_char_p, C.c_size_t]
+        lib.cs_io_free_geometry.argtypes = [C.POINTER(CGeometry)]
+        lib.cs_io_free_geometry.restype = None
+        lib.cs_io_read_species.argtypes = [C.c_char_p, C.POINTER(CSpeciesModel), C.c_char_p, C.c_size_t]
+        lib.cs_io_write_geometry.argtypes = [C.c_char_p, C.POINTER(CGeometry), C.c_char_p, C.c_size_t]
+        lib.cs_io_write_species.argtypes = [C.c_char_p, C.POINTER(CSpeciesModel), C.c_char_p, C.c_size_t]
+        lib.cs_io_read_dataset.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(C.c_double), C.c_uint64,
+                                           C.POINTER(C.c_uint64), C.POINTER(C.c_int)]
+        lib.cs_io_read_attr.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(C.c_double), C.c_uint64,
+                                        C.POINTER(C.c_uint64)]
+        lib.cs_io_read_attr_str.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_size_t]
         lib._cs_typed = True
     return lib
 
@@ -491,6 +521,119 @@ def options_to_c(o: Options) -> COptions:
     return co
 
 
+def _s(b) -> str:
+    return b.decode() if isinstance(b, bytes) else b.value.decode()
+
+
+def geometry_from_c(cg: CGeometry) -> Geometry:
+    N = tuple(int(v) for v in cg.N)
+    vox = np.ctypeslib.as_array(cg.voxels, shape=(int(np.prod(N)),)).reshape(N).copy()
+    nd = int(cg.num_domains)
+    conn = np.array([cg.domain_connections[i] for i in range(nd * nd)], dtype=np.uint8).reshape(nd, nd)
+    bnds = []
+    for i in range(int(cg.num_boundaries)):
+        n = int(cg.boundary_sizes[i])
+        arr = (np.ctypeslib.as_array(cg.boundaries[i], shape=(n * 6,)).reshape(n, 6).copy()
+               if n else np.zeros((0, 6), np.uint16))
+        bnds.append(Boundary(_s(cg.boundary_names[i]),
+                             "discrete" if cg.boundary_types[i] == BOUNDARY_DISCRETE else "density", arr))
+    return Geometry(voxels=vox, h=cg.h, domain_names=[_s(cg.domain_names[i]) for i in range(nd)],
+                    domain_connections=conn, boundaries=bnds, global_size=tuple(cg.global_size))
+
+
+def _channel_from_c(c: CChannelModel) -> ChannelModel:
+    return ChannelModel(c.Kd_open, c.k_min_open, c.k_max_open, c.n_open, c.Kd_close, c.k_min_close,
+                        c.k_max_close, c.n_close, c.i_unitary, _s(c.boundary), _s(c.species))
+
+
+def model_from_c(cm: CSpeciesModel) -> SpeciesModel:
+    nd, ns = int(cm.num_domains), int(cm.num_species)
+    diffusive = [[int(cm.diffusive[d][k]) for k in range(cm.num_diffusive[d])] for d in range(nd)]
+    sigma = [[float(cm.sigma[d][k]) for k in range(cm.num_diffusive[d])] for d in range(nd)]
+    init = np.array([[cm.init[d][s] for s in range(ns)] for d in range(nd)])
+    buffers = [[Buffer(int(cm.buffer_species[d][b][0]), int(cm.buffer_species[d][b][1]), cm.k_on[d][b],
+                       cm.k_off[d][b], cm.tot[d][b]) for b in range(cm.num_buffers[d])] for d in range(nd)]
+    return SpeciesModel(
+        domain_names=[_s(cm.domain_names[d]) for d in range(nd)],
+        species_names=[_s(cm.species_names[s]) for s in range(ns)],
+        diffusive=diffusive, sigma=sigma, init=init, buffers=buffers,
+        ryr=_channel_from_c(cm.ryr) if cm.use_ryr else None,
+        p_ryr=_channel_from_c(cm.p_ryr) if cm.use_p_ryr else None,
+        serca=SercaModel(cm.serca.v_cyt_to_A_sr, cm.serca.density, cm.serca.scale, _s(cm.serca.boundary),
+                         _s(cm.serca.species)) if cm.use_serca else None,
+        conv=tuple(cm.conv), fixed=[(int(cm.fixed[2 * i]), int(cm.fixed[2 * i + 1])) for i in range(cm.num_fixed)])
+
+
+def read_geometry(path: str) -> Geometry:
+    """Read a reference geometry file (HDF5)."""
+    cg = CGeometry()
+    err = C.create_string_buffer(512)
+    if io_lib().cs_io_read_geometry(path.encode(), C.byref(cg), err, 512) != 0:
+        raise CSError(err.value.decode())
+    try:
+        return geometry_from_c(cg)
+    finally:
+        io_lib().cs_io_free_geometry(C.byref(cg))
+
+
+def read_species(path: str) -> SpeciesModel:
+    """Read a reference species file (HDF5)."""
+    cm = CSpeciesModel()
+    err = C.create_string_buffer(512)
+    if io_lib().cs_io_read_species(path.encode(), C.byref(cm), err, 512) != 0:
+        raise CSError(err.value.decode())
+    return model_from_c(cm)
+
+
+def read_output_dataset(path: str, dataset: str, max_elems: int = 1 << 28) -> np.ndarray:
+    """A float64 dataset of a calcium_sparks output file."""
+    out = np.empty(max_elems if max_elems < (1 << 16) else 0, dtype=np.float64)
+    dims = (C.c_uint64 * 3)()
+    rank = C.c_int()
+    # query the size with a zero-length read first
+    probe = io_lib().cs_io_read_dataset(path.encode(), dataset.encode(), None, 0, dims, C.byref(rank))
+    n = int(np.prod([dims[i] for i in range(rank.value)])) if rank.value else 0
+    if rank.value == 0:
+        raise CSError(f"no dataset {dataset} in {path}")
+    out = np.empty(max(n, 1), dtype=np.float64)
+    if io_lib().cs_io_read_dataset(path.encode(), dataset.encode(), out.ctypes.data_as(C.POINTER(C.c_double)),
+                                   n, dims, C.byref(rank)) != 0:
+        raise CSError(f"cannot read {dataset} from {path}")
+    del probe
+    return out[:n].reshape([dims[i] for i in range(rank.value)])
+
+
+def read_output_attr(path: str, obj: str, attr: str) -> np.ndarray:
+    out = np.empty(64, dtype=np.float64)
+    cnt = C.c_uint64()
+    if io_lib().cs_io_read_attr(path.encode(), obj.encode(), attr.encode(),
+                                out.ctypes.data_as(C.POINTER(C.c_double)), 64, C.byref(cnt)) != 0:
+        raise CSError(f"no attribute {obj}:{attr} in {path}")
+    return out[:cnt.value]
+
+
+def read_output_attr_str(path: str, obj: str, attr: str) -> str:
+    buf = C.create_string_buffer(512)
+    if io_lib().cs_io_read_attr_str(path.encode(), obj.encode(), attr.encode(), buf, 512) != 0:
+        raise CSError(f"no attribute {obj}:{attr} in {path}")
+    return buf.value.decode()
+
+
+def write_geometry(path: str, g: Geometry):
+    hold = _Holder()
+    cg = geometry_to_c(g, hold)
+    err = C.create_string_buffer(512)
+    if io_lib().cs_io_write_geometry(path.encode(), C.byref(cg), err, 512) != 0:
+        raise CSError(err.value.decode())
+
+
+def write_species(path: str, m: SpeciesModel):
+    cm = model_to_c(m)
+    err = C.create_string_buffer(512)
+    if io_lib().cs_io_write_species(path.encode(), C.byref(cm), err, 512) != 0:
+        raise CSError(err.value.decode())
+
+
 class Setup:
     """cs_setup_build result: the derived problem plus host metadata."""
 
@@ -607,6 +750,23 @@ class Simulation:
         _check(device_lib().cs_set_channel_states(self.ctx, flux, arr.ctypes.data_as(C.POINTER(C.c_uint8))),
                "cs_set_channel_states")
 
+    def launch_config(self) -> dict:
+        cfg = CLaunchConfig()
+        _check(device_lib().cs_get_launch_config(self.ctx, C.byref(cfg)), "cs_get_launch_config")
+        return cfg.as_dict()
+
+    def set_launch_config(self, **kw):
+        cfg = CLaunchConfig(**{**self.launch_config(), **kw})
+        _check(device_lib().cs_set_launch_config(self.ctx, C.byref(cfg)), "cs_set_launch_config")
+
+    def autotune(self, reps: int = 1):
+        """Pick the fastest launch shape of the stencil kernel for this grid
+        over the coming steps' substep pattern (the state is not advanced);
+        returns (config, mean kernel ms)."""
+        cfg, ms = CLaunchConfig(), C.c_double()
+        _check(device_lib().cs_autotune(self.ctx, self.time_ind, reps, C.byref(cfg), C.byref(ms)), "cs_autotune")
+        return cfg.as_dict(), ms.value
+
     def time_steps(self, nsteps: int):
         """Time nsteps steps with HIP events; returns (total_ms, mean stencil kernel ms)."""
         tot, ker = C.c_double(), C.c_double()
@@ -633,6 +793,13 @@ class Simulation:
             self.close()
         except Exception:
             pass
+
+
+def calibration_copy(nbytes: int, reps: int) -> float:
+    """Plain device copy of nbytes, reps times; returns ms per copy."""
+    ms = C.c_double()
+    _check(device_lib().cs_calibration_copy(nbytes, reps, C.byref(ms)), "cs_calibration_copy")
+    return ms.value
 
 
 def device_info() -> str:

@@ -44,7 +44,6 @@ constexpr int MAXS = CS_MAX_SPECIES;
 constexpr int MAXD = CS_MAX_DOMAINS;
 constexpr int MAXF = CS_MAX_FLUXES;
 constexpr int BX = 64;          /* one wavefront along x                  */
-constexpr int BY = 4;           /* four wavefronts per block along y      */
 constexpr int GATE_THREADS = 256;
 constexpr int FLUX_THREADS = 256;
 constexpr int STAT_THREADS = 256;
@@ -101,10 +100,28 @@ struct Geo {
 
 /* ------------------------------------------------------------ kernels */
 
-template <typename T, int NS>
-__global__ void __launch_bounds__(BX * BY)
+/* Launch shape of k_stencil, chosen by cs_autotune (or the default). */
+struct LaunchCfg {
+  int rows;      /* wavefronts per block along y (block = 64 x rows)       */
+  int zchunk;    /* planes marched per block                                */
+  int xcd;       /* 1: blocks b, b+8, b+16.. (one XCD) get adjacent tiles  */
+  int pf;        /* planes loaded ahead of the one being computed (1 or 2)  */
+  int nt;        /* 1: non-temporal stores of the next state               */
+  int lds;       /* 1: x/y neighbours from an LDS plane tile                */
+};
+
+template <typename T>
+__device__ inline void store_next(T* p, T v, bool nt)
+{
+  if (nt) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
+template <typename T, int NS, int PF, bool NT>
+__global__ void __launch_bounds__(BX * 8)
 k_stencil(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__ ids,
-          const DevParams<T>* __restrict__ P, Geo g, uint32_t dmask, uint32_t react, int zchunk)
+          const DevParams<T>* __restrict__ P, Geo g, uint32_t dmask, uint32_t react,
+          int zchunk, int gx, int gy, int nblocks, int xcd)
 {
   __shared__ T s_coef[NS][MAXD * MAXD];
   __shared__ T s_kon[MAXS][MAXD], s_koff[MAXS][MAXD], s_tot[MAXS][MAXD];
@@ -112,9 +129,21 @@ k_stencil(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict
   __shared__ uint8_t s_b0[MAXS][MAXD], s_b1[MAXS][MAXD];
   __shared__ uint32_t s_fixed[MAXD];
 
+  /* tile of this block; with xcd set, the hardware's round-robin of blocks
+   * over the 8 XCDs hands each XCD a contiguous run of tiles, so halo rows
+   * shared by neighbouring tiles hit in that XCD's L2 */
+  int bid = blockIdx.x;
+  if (xcd) {
+    const int per = (nblocks + 7) / 8;
+    bid = (bid % 8) * per + bid / 8;
+  }
+  if (bid >= nblocks) return;
+  const int bx = bid % gx, by = (bid / gx) % gy, bz = bid / (gx * gy);
+
+  const int nthr = BX * blockDim.y;
   const int tid = threadIdx.y * BX + threadIdx.x;
-  for (int i = tid; i < NS * MAXD * MAXD; i += BX * BY) s_coef[i / (MAXD * MAXD)][i % (MAXD * MAXD)] = P->coef[i / (MAXD * MAXD)][i % (MAXD * MAXD)];
-  for (int i = tid; i < MAXS * MAXD; i += BX * BY) {
+  for (int i = tid; i < NS * MAXD * MAXD; i += nthr) s_coef[i / (MAXD * MAXD)][i % (MAXD * MAXD)] = P->coef[i / (MAXD * MAXD)][i % (MAXD * MAXD)];
+  for (int i = tid; i < MAXS * MAXD; i += nthr) {
     const int b = i / MAXD, d = i % MAXD;
     s_kon[b][d] = P->kon[b][d];
     s_koff[b][d] = P->koff[b][d];
@@ -122,14 +151,14 @@ k_stencil(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict
     s_b0[b][d] = P->bsp0[b][d];
     s_b1[b][d] = P->bsp1[b][d];
   }
-  for (int i = tid; i < MAXD * NS; i += BX * BY) s_init[i / NS][i % NS] = P->init[i / NS][i % NS];
+  for (int i = tid; i < MAXD * NS; i += nthr) s_init[i / NS][i % NS] = P->init[i / NS][i % NS];
   if (tid < MAXD) s_fixed[tid] = P->fixed_mask[tid];
   __syncthreads();
 
-  const int x = blockIdx.x * BX + threadIdx.x;
-  const int y = blockIdx.y * BY + threadIdx.y;
+  const int x = bx * BX + threadIdx.x;
+  const int y = by * (int)blockDim.y + threadIdx.y;
   if (x >= g.nx || y >= g.ny) return;
-  const int z0 = blockIdx.z * zchunk;
+  const int z0 = bz * zchunk;
   const int z1 = min(g.nz, z0 + zchunk);
   if (z0 >= z1) return;
 
@@ -142,21 +171,34 @@ k_stencil(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict
   const int64_t oym = y > 0 ? -(int64_t)g.nx : 0, oyp = y + 1 < g.ny ? (int64_t)g.nx : 0;
   const bool xyface = x == 0 || y == 0 || x + 1 == g.nx || y + 1 == g.ny;
   const uint64_t col = (uint64_t)y * g.nx + x;
+  const int ztop_plane = g.nz - 1;
 
-  T um[NS], uc[NS], up[NS];
+  /* register ring over z: um = z-1, uc = z, up = z+1, upp = z+2 (PF 2);
+   * plane indices are clamped, so a missing z neighbour equals uc */
+  T um[NS], uc[NS], up[NS], upp[NS];
   {
-    const uint64_t v = (uint64_t)z0 * g.plane + col;
+    const uint64_t vc = (uint64_t)z0 * g.plane + col;
+    const uint64_t vm = (uint64_t)max(z0 - 1, 0) * g.plane + col;
+    const uint64_t vp = (uint64_t)min(z0 + 1, ztop_plane) * g.plane + col;
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-      uc[s] = u[s * g.nv + v];
-      um[s] = z0 > 0 ? u[s * g.nv + v - g.plane] : uc[s];
+      uc[s] = u[s * g.nv + vc];
+      um[s] = u[s * g.nv + vm];
+      if (PF == 2) up[s] = u[s * g.nv + vp];
     }
   }
   for (int z = z0; z < z1; z++) {
     const uint64_t v = (uint64_t)z * g.plane + col;
-    const bool ztop = z + 1 < g.nz;
+    const bool ztop = z < ztop_plane;
+    if (PF == 1) {
+      const uint64_t vp = (uint64_t)min(z + 1, ztop_plane) * g.plane + col;
 #pragma unroll
-    for (int s = 0; s < NS; s++) up[s] = ztop ? u[s * g.nv + v + g.plane] : uc[s];
+      for (int s = 0; s < NS; s++) up[s] = u[s * g.nv + vp];
+    } else if (z + 2 <= z1) {
+      const uint64_t vpp = (uint64_t)min(z + 2, ztop_plane) * g.plane + col;
+#pragma unroll
+      for (int s = 0; s < NS; s++) upp[s] = u[s * g.nv + vpp];
+    }
 
     const uint8_t id = ids[v];
     const int d = id & 7;
@@ -217,10 +259,198 @@ k_stencil(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict
     }
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-      un[s * g.nv + v] = nw[s];
+      store_next(un + s * g.nv + v, nw[s], NT);
+      um[s] = uc[s];
+      uc[s] = up[s];
+      if (PF == 2) up[s] = upp[s];
+    }
+  }
+}
+
+/* LDS-tiled variant: the block's 64 x rows columns of plane z, with a
+ * one-voxel halo, are staged in a double-buffered LDS tile, so every value
+ * is loaded from memory once per plane (plus the halo) and the x/y
+ * neighbours come from LDS; z neighbours stay in the register ring.  Halo
+ * coordinates are clamped to the grid, which makes a missing neighbour
+ * equal to the voxel itself (no-flux faces).  Threads past the grid edge
+ * keep participating in the barriers but store nothing. */
+template <typename T, int NS, bool NT>
+__global__ void __launch_bounds__(BX * 8)
+k_stencil_lds(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__ ids,
+              const DevParams<T>* __restrict__ P, Geo g, uint32_t dmask, uint32_t react,
+              int zchunk, int gx, int gy, int nblocks, int xcd)
+{
+  constexpr int TW = BX + 2;
+  extern __shared__ unsigned char lds_raw[];
+  __shared__ T s_coef[NS][MAXD * MAXD];
+  __shared__ T s_kon[MAXS][MAXD], s_koff[MAXS][MAXD], s_tot[MAXS][MAXD];
+  __shared__ T s_init[MAXD][NS];
+  __shared__ uint8_t s_b0[MAXS][MAXD], s_b1[MAXS][MAXD];
+  __shared__ uint32_t s_fixed[MAXD];
+
+  int bid = blockIdx.x;
+  if (xcd) {
+    const int per = (nblocks + 7) / 8;
+    bid = (bid % 8) * per + bid / 8;
+  }
+  if (bid >= nblocks) return;
+  const int rows = (int)blockDim.y;
+  const int bx = bid % gx, by = (bid / gx) % gy, bz = bid / (gx * gy);
+  const int nthr = BX * rows;
+  const int tid = threadIdx.y * BX + threadIdx.x;
+  for (int i = tid; i < NS * MAXD * MAXD; i += nthr) s_coef[i / (MAXD * MAXD)][i % (MAXD * MAXD)] = P->coef[i / (MAXD * MAXD)][i % (MAXD * MAXD)];
+  for (int i = tid; i < MAXS * MAXD; i += nthr) {
+    const int b = i / MAXD, d = i % MAXD;
+    s_kon[b][d] = P->kon[b][d];
+    s_koff[b][d] = P->koff[b][d];
+    s_tot[b][d] = P->tot[b][d];
+    s_b0[b][d] = P->bsp0[b][d];
+    s_b1[b][d] = P->bsp1[b][d];
+  }
+  for (int i = tid; i < MAXD * NS; i += nthr) s_init[i / NS][i % NS] = P->init[i / NS][i % NS];
+  if (tid < MAXD) s_fixed[tid] = P->fixed_mask[tid];
+
+  /* tile[buf][s][row 0..rows+1][col 0..65] */
+  T* tile = reinterpret_cast<T*>(lds_raw);
+  const int tstride = (rows + 2) * TW;        /* one species plane */
+  const int bstride = NS * tstride;           /* one buffer         */
+
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int x = bx * BX + tx, y = by * rows + ty;
+  const bool inside = x < g.nx && y < g.ny;
+  const int xc = min(x, g.nx - 1), yc = min(y, g.ny - 1);
+  const int z0 = bz * zchunk;
+  const int z1 = min(g.nz, z0 + zchunk);
+  const uint64_t col = (uint64_t)yc * g.nx + xc;
+  /* halo cells this thread loads: an x halo (lanes 0 and 63) and a y halo
+   * (first and last wave of the block), at clamped coordinates */
+  const bool hx = tx == 0 || tx == BX - 1;
+  const int hxc = tx == 0 ? max(x - 1, 0) : min(x + 1, g.nx - 1);
+  const uint64_t hxcol = (uint64_t)yc * g.nx + hxc;
+  const int hxslot = (ty + 1) * TW + (tx == 0 ? 0 : TW - 1);
+  const bool hy = ty == 0 || ty == rows - 1;
+  const int hyc = ty == 0 ? max(y - 1, 0) : min(y + 1, g.ny - 1);
+  const uint64_t hycol = (uint64_t)hyc * g.nx + xc;
+  const int hyslot = (ty == 0 ? 0 : rows + 1) * TW + tx + 1;
+  const int myslot = (ty + 1) * TW + tx + 1;
+  /* rows == 1: the single wave loads both y halos */
+  const bool hy2 = rows == 1;
+  const uint64_t hy2col = (uint64_t)min(y + 1, g.ny - 1) * g.nx + xc;
+  const int hy2slot = 2 * TW + tx + 1;
+
+  const int nb = react ? (int)P->nb : 0;
+  const T dtr = P->dt_react;
+  const bool dirichlet = P->dirichlet != 0;
+  const bool xyface = x == 0 || y == 0 || x + 1 == g.nx || y + 1 == g.ny;
+  const int ztop_plane = g.nz - 1;
+
+  T um[NS], uc[NS], up[NS];
+  T hxv[NS], hyv[NS], hy2v[NS];
+  {
+    const uint64_t pc = (uint64_t)z0 * g.plane, pm = (uint64_t)max(z0 - 1, 0) * g.plane;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      uc[s] = u[s * g.nv + pc + col];
+      um[s] = u[s * g.nv + pm + col];
+      if (hx) hxv[s] = u[s * g.nv + pc + hxcol];
+      if (hy) hyv[s] = u[s * g.nv + pc + hycol];
+      if (hy2) hy2v[s] = u[s * g.nv + pc + hy2col];
+    }
+  }
+  int buf = 0;
+  for (int z = z0; z < z1; z++) {
+    const uint64_t pz = (uint64_t)z * g.plane;
+    const uint64_t v = pz + col;
+    const bool ztop = z < ztop_plane;
+    /* stage plane z */
+    T* tb = tile + buf * bstride;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      tb[s * tstride + myslot] = uc[s];
+      if (hx) tb[s * tstride + hxslot] = hxv[s];
+      if (hy) tb[s * tstride + hyslot] = hyv[s];
+      if (hy2) tb[s * tstride + hy2slot] = hy2v[s];
+    }
+    /* prefetch plane z+1 (clamped) */
+    {
+      const uint64_t pp = (uint64_t)min(z + 1, ztop_plane) * g.plane;
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        up[s] = u[s * g.nv + pp + col];
+        if (z + 1 < z1) {
+          if (hx) hxv[s] = u[s * g.nv + pp + hxcol];
+          if (hy) hyv[s] = u[s * g.nv + pp + hycol];
+          if (hy2) hy2v[s] = u[s * g.nv + pp + hy2col];
+        }
+      }
+    }
+    const uint8_t id = ids[v];
+    __syncthreads();
+    if (inside) {
+      const int d = id & 7;
+      const bool uni = (id & 0x80) != 0;
+      int dxm = d, dxp = d, dym = d, dyp = d, dzm = d, dzp = d;
+      if (!uni) {
+        dxm = ids[v - (x > 0 ? 1 : 0)] & 7;
+        dxp = ids[v + (x + 1 < g.nx ? 1 : 0)] & 7;
+        dym = ids[v - (y > 0 ? (uint64_t)g.nx : 0)] & 7;
+        dyp = ids[v + (y + 1 < g.ny ? (uint64_t)g.nx : 0)] & 7;
+        dzm = z > 0 ? ids[v - g.plane] & 7 : d;
+        dzp = ztop ? ids[v + g.plane] & 7 : d;
+      }
+      T nw[NS];
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        T acc = uc[s];
+        if ((dmask >> s) & 1u) {
+          const T* ts = tb + s * tstride + myslot;
+          const T c = uc[s];
+          const T exm = ts[-1] - c, exp_ = ts[1] - c;
+          const T eym = ts[-TW] - c, eyp = ts[TW] - c;
+          const T ezm = um[s] - c, ezp = up[s] - c;
+          const T* C = s_coef[s] + d * MAXD;
+          if (uni) {
+            acc = c + C[d] * ((ezm + ezp) + (eym + eyp) + (exm + exp_));
+          } else {
+            acc = c + ((C[dzm] * ezm + C[dzp] * ezp) + (C[dym] * eym + C[dyp] * eyp) +
+                       (C[dxm] * exm + C[dxp] * exp_));
+          }
+        }
+        nw[s] = acc;
+      }
+      for (int b = 0; b < nb; b++) {
+        const int b0 = s_b0[b][d], b1 = s_b1[b][d];
+        T B0 = uc[0], B1 = uc[0];
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+          if (s == b0) B0 = uc[s];
+          if (s == b1) B1 = uc[s];
+        }
+        const T J = s_kon[b][d] * (s_tot[b][d] - B0) * B1 - s_koff[b][d] * B0;
+        const T inc = dtr * J;
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+          if (s == b0) nw[s] = (((dmask >> s) & 1u) ? nw[s] : uc[s]) + inc;
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+          if (s == b1) nw[s] = (((dmask >> s) & 1u) ? nw[s] : uc[s]) - inc;
+      }
+      if (dirichlet && (xyface || z == 0 || !ztop)) nw[0] = s_init[d][0];
+      const uint32_t fm = s_fixed[d];
+      if (fm) {
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+          if ((fm >> s) & 1u) nw[s] = s_init[d][s];
+      }
+#pragma unroll
+      for (int s = 0; s < NS; s++) store_next(un + s * g.nv + v, nw[s], NT);
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
       um[s] = uc[s];
       uc[s] = up[s];
     }
+    buf ^= 1;
   }
 }
 
@@ -387,6 +617,15 @@ __global__ void k_stats_final(const double* __restrict__ partial, uint32_t nbloc
   out[3 * i + 0] = a; out[3 * i + 1] = b; out[3 * i + 2] = c;
 }
 
+/* Plain copy of a known number of bytes (16 B per lane per iteration),
+ * used to calibrate memory-traffic counters against algorithmic bytes. */
+__global__ void k_calibration_copy(const uint4* __restrict__ a, uint4* __restrict__ b, uint64_t n16)
+{
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
 /* Fill every voxel with init[domain][species]. */
 template <typename T>
 __global__ void k_fill_init(T* __restrict__ u, const uint8_t* __restrict__ ids,
@@ -438,10 +677,31 @@ struct cs_context {
   std::vector<uint8_t> init_states;
   uint32_t substeps[MAXS] = {0};
   uint32_t flux_substep[MAXF] = {0};
-  int zchunk = 1, zsplit = 1;
+  LaunchCfg launch{4, 32, 1, 1, 0, 0};
 };
 
 namespace {
+
+/* Dynamic LDS of the tiled kernel: two buffers of ns planes of
+ * (rows + 2) x 66 values; kept under 60 KiB so the default per-block limit
+ * holds with the static parameter tables. */
+size_t lds_tile_bytes(size_t esize, uint32_t ns, int rows)
+{
+  return esize * 2 * ns * (size_t)(rows + 2) * (BX + 2);
+}
+constexpr size_t LDS_TILE_LIMIT = 60 * 1024;
+
+/* Default launch shape: enough blocks to keep every CU busy several
+ * times over, z chunks no shorter than a few planes. */
+LaunchCfg default_launch(const Geo& g)
+{
+  LaunchCfg L{4, 32, 1, 1, 0, 0};
+  const uint64_t xy = (uint64_t)((g.nx + BX - 1) / BX) * ((g.ny + L.rows - 1) / L.rows);
+  int chunks = (int)std::max<uint64_t>(1, (4096 + xy - 1) / xy);
+  L.zchunk = std::max(4, (g.nz + chunks - 1) / chunks);
+  L.zchunk = std::min(L.zchunk, g.nz);
+  return L;
+}
 
 void free_ctx(cs_context* c)
 {
@@ -584,29 +844,60 @@ int fill_init(cs_context* c)
   return CS_OK;
 }
 
-template <typename T, int NS>
-void launch_stencil_ns(cs_context* c, uint32_t dmask, uint32_t react)
+template <typename T, int NS, int PF, bool NT>
+void launch_stencil_v(cs_context* c, const LaunchCfg& L, const void* uin, void* uout, uint32_t dmask,
+                      uint32_t react)
 {
   const Geo& g = c->geo;
-  dim3 grid((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, c->zsplit);
-  hipLaunchKernelGGL((k_stencil<T, NS>), grid, dim3(BX, BY), 0, c->stream,
-                     (const T*)c->d_u[c->cur], (T*)c->d_u[c->cur ^ 1], c->d_ids,
-                     (const DevParams<T>*)c->d_params, g, dmask, react, c->zchunk);
+  const int gx = (g.nx + BX - 1) / BX, gy = (g.ny + L.rows - 1) / L.rows;
+  const int gz = (g.nz + L.zchunk - 1) / L.zchunk;
+  const int nblocks = gx * gy * gz;
+  const int launch = L.xcd ? ((nblocks + 7) / 8) * 8 : nblocks;
+  if (L.lds) {
+    const size_t lds = sizeof(T) * 2 * NS * (L.rows + 2) * (BX + 2);
+    hipLaunchKernelGGL((k_stencil_lds<T, NS, NT>), dim3(launch), dim3(BX, L.rows), lds, c->stream,
+                       (const T*)uin, (T*)uout, c->d_ids, (const DevParams<T>*)c->d_params, g, dmask,
+                       react, L.zchunk, gx, gy, nblocks, L.xcd);
+  } else {
+    hipLaunchKernelGGL((k_stencil<T, NS, PF, NT>), dim3(launch), dim3(BX, L.rows), 0, c->stream,
+                       (const T*)uin, (T*)uout, c->d_ids, (const DevParams<T>*)c->d_params, g, dmask,
+                       react, L.zchunk, gx, gy, nblocks, L.xcd);
+  }
+}
+
+template <typename T, int NS>
+void launch_stencil_ns(cs_context* c, const LaunchCfg& L, const void* uin, void* uout, uint32_t dmask,
+                       uint32_t react)
+{
+  if (L.pf == 2 && !L.lds) {
+    if (L.nt) launch_stencil_v<T, NS, 2, true>(c, L, uin, uout, dmask, react);
+    else launch_stencil_v<T, NS, 2, false>(c, L, uin, uout, dmask, react);
+  } else {
+    if (L.nt) launch_stencil_v<T, NS, 1, true>(c, L, uin, uout, dmask, react);
+    else launch_stencil_v<T, NS, 1, false>(c, L, uin, uout, dmask, react);
+  }
+}
+
+template <typename T>
+void launch_stencil_cfg(cs_context* c, const LaunchCfg& L, const void* uin, void* uout, uint32_t dmask,
+                        uint32_t react)
+{
+  switch (c->ns) {
+    case 1: launch_stencil_ns<T, 1>(c, L, uin, uout, dmask, react); break;
+    case 2: launch_stencil_ns<T, 2>(c, L, uin, uout, dmask, react); break;
+    case 3: launch_stencil_ns<T, 3>(c, L, uin, uout, dmask, react); break;
+    case 4: launch_stencil_ns<T, 4>(c, L, uin, uout, dmask, react); break;
+    case 5: launch_stencil_ns<T, 5>(c, L, uin, uout, dmask, react); break;
+    case 6: launch_stencil_ns<T, 6>(c, L, uin, uout, dmask, react); break;
+    case 7: launch_stencil_ns<T, 7>(c, L, uin, uout, dmask, react); break;
+    default: launch_stencil_ns<T, 8>(c, L, uin, uout, dmask, react); break;
+  }
 }
 
 template <typename T>
 void launch_stencil(cs_context* c, uint32_t dmask, uint32_t react)
 {
-  switch (c->ns) {
-    case 1: launch_stencil_ns<T, 1>(c, dmask, react); break;
-    case 2: launch_stencil_ns<T, 2>(c, dmask, react); break;
-    case 3: launch_stencil_ns<T, 3>(c, dmask, react); break;
-    case 4: launch_stencil_ns<T, 4>(c, dmask, react); break;
-    case 5: launch_stencil_ns<T, 5>(c, dmask, react); break;
-    case 6: launch_stencil_ns<T, 6>(c, dmask, react); break;
-    case 7: launch_stencil_ns<T, 7>(c, dmask, react); break;
-    default: launch_stencil_ns<T, 8>(c, dmask, react); break;
-  }
+  launch_stencil_cfg<T>(c, c->launch, c->d_u[c->cur], c->d_u[c->cur ^ 1], dmask, react);
 }
 
 /* One step: gate (if due), stencil+reaction, fluxes; then swap. */
@@ -864,13 +1155,7 @@ int cs_create(const cs_problem* p, cs_context** out)
   if (hipMalloc(&c->d_partial, sizeof(double) * STAT_BLOCKS * MAXS * MAXD * 3) != hipSuccess ||
       hipMalloc(&c->d_stats, sizeof(double) * MAXS * MAXD * 3) != hipSuccess)
     return bail(set_err(CS_ENOMEM, "device allocation of reduction buffers failed"));
-  /* z split: enough blocks to cover the 256 CUs several times over */
-  const uint64_t xy_blocks = (uint64_t)((g.nx + BX - 1) / BX) * ((g.ny + BY - 1) / BY);
-  int zs = (int)std::max<uint64_t>(1, (2048 + xy_blocks - 1) / xy_blocks);
-  zs = std::min(zs, std::max(1, g.nz / 8));
-  zs = std::max(1, std::min(zs, g.nz));
-  c->zchunk = (g.nz + zs - 1) / zs;
-  c->zsplit = (g.nz + c->zchunk - 1) / c->zchunk;
+  c->launch = default_launch(g);
   rc = c->esize == 8 ? fill_init<double>(c) : fill_init<float>(c);
   if (rc != CS_OK) return bail(rc);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(set_err(CS_EDEVICE, "setup failed"));
@@ -1034,6 +1319,148 @@ int cs_set_channel_states(cs_context* c, uint32_t f, const uint8_t* states)
     HIP_TRY(hipMemcpyAsync(c->d_states + c->state_off[f], tmp.data(), c->npairs[f], hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return CS_OK;
+}
+
+int cs_get_launch_config(const cs_context* c, cs_launch_config* out)
+{
+  if (!c || !out) return set_err(CS_EINVAL, "NULL argument");
+  out->rows = (uint32_t)c->launch.rows;
+  out->zchunk = (uint32_t)c->launch.zchunk;
+  out->xcd_remap = (uint32_t)c->launch.xcd;
+  out->prefetch = (uint32_t)c->launch.pf;
+  out->nontemporal = (uint32_t)c->launch.nt;
+  out->lds_tile = (uint32_t)c->launch.lds;
+  return CS_OK;
+}
+
+int cs_set_launch_config(cs_context* c, const cs_launch_config* in)
+{
+  if (!c || !in) return set_err(CS_EINVAL, "NULL argument");
+  if (!(in->rows == 1 || in->rows == 2 || in->rows == 4 || in->rows == 8))
+    return set_err(CS_EINVAL, "rows must be 1, 2, 4 or 8");
+  if (in->zchunk < 1) return set_err(CS_EINVAL, "zchunk must be >= 1");
+  if (in->prefetch != 1 && in->prefetch != 2) return set_err(CS_EINVAL, "prefetch must be 1 or 2");
+  if (in->xcd_remap > 1 || in->nontemporal > 1 || in->lds_tile > 1) return set_err(CS_EINVAL, "flags must be 0 or 1");
+  if (in->lds_tile && in->prefetch != 1) return set_err(CS_EINVAL, "the LDS tile kernel prefetches one plane");
+  if (in->lds_tile && lds_tile_bytes(c->esize, c->ns, (int)in->rows) > LDS_TILE_LIMIT)
+    return set_err(CS_EINVAL, "LDS tile of %u rows too large for %u species", in->rows, c->ns);
+  LaunchCfg L;
+  L.rows = (int)in->rows;
+  L.zchunk = (int)std::min<uint32_t>(in->zchunk, (uint32_t)c->geo.nz);
+  L.xcd = (int)in->xcd_remap;
+  L.pf = (int)in->prefetch;
+  L.nt = (int)in->nontemporal;
+  L.lds = (int)in->lds_tile;
+  c->launch = L;
+  return CS_OK;
+}
+
+int cs_autotune(cs_context* c, uint64_t first_time_ind, uint32_t reps, cs_launch_config* best,
+                double* best_ms)
+{
+  if (!c) return set_err(CS_EINVAL, "context is NULL");
+  if (first_time_ind == 0) return set_err(CS_EINVAL, "time indices start at 1");
+  if (reps < 1) reps = 1;
+  HIP_TRY(hipSetDevice(c->device));
+  /* the substep pattern of the coming steps: which species diffuse and
+   * whether the reactions fire differs from step to step, and so does the
+   * best shape, so every candidate is timed over the same run of steps */
+  uint64_t period = 1;
+  auto lcm = [](uint64_t a, uint64_t b) { uint64_t x = a, y = b; while (y) { uint64_t t = x % y; x = y; y = t; } return a / x * b; };
+  for (uint32_t s = 0; s < c->ns; s++)
+    if (c->substeps[s]) period = std::min<uint64_t>(lcm(period, c->substeps[s]), 240);
+  if (c->prob.num_buffers) period = std::min<uint64_t>(lcm(period, c->prob.reaction_substep), 240);
+  std::vector<uint32_t> masks, reacts;
+  for (uint64_t k = 0; k < period * reps; k++) {
+    const uint64_t ti = first_time_ind + k;
+    uint32_t dm = 0;
+    for (uint32_t s = 0; s < c->ns; s++)
+      if (c->substeps[s] && ti % c->substeps[s] == 0) dm |= 1u << s;
+    masks.push_back(dm);
+    reacts.push_back((c->prob.num_buffers > 0 && ti % c->prob.reaction_substep == 0) ? 1u : 0u);
+  }
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  LaunchCfg winner = c->launch;
+  float wms = 1e30f;
+  const int nz = c->geo.nz;
+  std::vector<int> zcs;
+  for (int zc : {8, 16, 32, 64})
+    if (zc < nz) zcs.push_back(zc);
+  zcs.push_back(nz);
+  int rc = CS_OK;
+  for (int lds : {0, 1})
+  for (int rows : {2, 4, 8})
+    for (int zc : zcs)
+      for (int xcd : {0, 1})
+        for (int pf : {1, 2})
+          for (int nt : {0, 1}) {
+            if (lds && pf != 1) continue;
+            if (lds && lds_tile_bytes(c->esize, c->ns, rows) > LDS_TILE_LIMIT) continue;
+            LaunchCfg L{rows, zc, xcd, pf, nt, lds};
+            /* reads the current state, writes the other buffer (scratch
+             * until the next step overwrites it); the state is unchanged */
+            auto go = [&](uint32_t dm, uint32_t re) {
+              if (c->esize == 8) launch_stencil_cfg<double>(c, L, c->d_u[c->cur], c->d_u[c->cur ^ 1], dm, re);
+              else launch_stencil_cfg<float>(c, L, c->d_u[c->cur], c->d_u[c->cur ^ 1], dm, re);
+            };
+            go(masks[0], reacts[0]);
+            if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = set_err(CS_EDEVICE, "event record failed"); goto done; }
+            for (size_t k = 0; k < masks.size(); k++) go(masks[k], reacts[k]);
+            if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+              rc = set_err(CS_EDEVICE, "autotune launch failed: %s", hipGetErrorString(hipGetLastError()));
+              goto done;
+            }
+            float ms = 0;
+            hipEventElapsedTime(&ms, e0, e1);
+            ms /= (float)masks.size();
+            if (ms < wms) { wms = ms; winner = L; }
+          }
+done:
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc != CS_OK) return rc;
+  HIP_TRY(hipGetLastError());
+  c->launch = winner;
+  if (best) cs_get_launch_config(c, best);
+  if (best_ms) *best_ms = wms;
+  return CS_OK;
+}
+
+int cs_calibration_copy(uint64_t bytes, uint32_t reps, double* ms_per_copy)
+{
+  if (bytes < 16 || reps < 1) return set_err(CS_EINVAL, "bytes >= 16 and reps >= 1 required");
+  const uint64_t n16 = bytes / 16;
+  void *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = CS_OK;
+  float ms = 0;
+  if (hipMalloc(&a, n16 * 16) != hipSuccess || hipMalloc(&b, n16 * 16) != hipSuccess) {
+    rc = set_err(CS_ENOMEM, "calibration buffers");
+    goto out;
+  }
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess) {
+    rc = set_err(CS_EDEVICE, "calibration stream");
+    goto out;
+  }
+  hipLaunchKernelGGL(k_calibration_copy, dim3(4096), dim3(256), 0, st, (const uint4*)a, (uint4*)b, n16);
+  hipEventRecord(e0, st);
+  for (uint32_t r = 0; r < reps; r++)
+    hipLaunchKernelGGL(k_calibration_copy, dim3(4096), dim3(256), 0, st, (const uint4*)a, (uint4*)b, n16);
+  hipEventRecord(e1, st);
+  if (hipEventSynchronize(e1) != hipSuccess) { rc = set_err(CS_EDEVICE, "calibration copy failed"); goto out; }
+  hipEventElapsedTime(&ms, e0, e1);
+  if (ms_per_copy) *ms_per_copy = ms / reps;
+out:
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (st) hipStreamDestroy(st);
+  if (a) hipFree(a);
+  if (b) hipFree(b);
+  return rc;
 }
 
 double cs_ryr_flux(double dt, double h, double u0, double u1, const double* params)

@@ -156,6 +156,32 @@ int cs_set_channel_states(cs_context* ctx, uint32_t flux, const uint8_t* states)
 int cs_time_steps(cs_context* ctx, uint64_t first_time_ind, uint64_t nsteps,
                   double* total_ms, double* stencil_kernel_ms);
 
+/* Launch shape of the fused stencil/reaction kernel: blocks of 64 x rows
+ * threads march zchunk planes; xcd_remap orders tiles so each of the 8
+ * XCDs works on a contiguous slab (shared L2 for halo rows); prefetch is
+ * the number of planes loaded ahead (1 or 2); nontemporal streams the next
+ * state past the caches; lds_tile stages each plane in LDS so the x/y
+ * neighbours are read from LDS instead of memory.  No reference counterpart: the reference's
+ * OpenMP/AVX-512 loop has no launch shape (kernel_generic.cpp:561-600). */
+typedef struct cs_launch_config {
+  uint32_t rows, zchunk, xcd_remap, prefetch, nontemporal, lds_tile;
+} cs_launch_config;
+
+int cs_get_launch_config(const cs_context* ctx, cs_launch_config* cfg);
+int cs_set_launch_config(cs_context* ctx, const cs_launch_config* cfg);
+
+/* Time every candidate launch shape on the context's grid over reps
+ * periods of the substep pattern starting at first_time_ind (reading the
+ * current state and writing the spare buffer, so the state is unchanged)
+ * and keep the fastest; best_ms is its mean kernel time. */
+int cs_autotune(cs_context* ctx, uint64_t first_time_ind, uint32_t reps,
+                cs_launch_config* best, double* best_ms);
+
+/* Copy `bytes` (rounded down to 16) between two scratch device buffers
+ * reps times with a plain copy kernel (k_calibration_copy); used to scale
+ * memory-traffic counters.  No reference counterpart. */
+int cs_calibration_copy(uint64_t bytes, uint32_t reps, double* ms_per_copy);
+
 /* Scalar flux functions of the plugin table, evaluated on the host with
  * the same code the device runs (ryr_flux boundaryfluxes.c:87-141,
  * serca_flux :12-85). */

@@ -1,6 +1,9 @@
 """Shared test inputs: small dyads the oracle steps in seconds."""
 from __future__ import annotations
 
+import json
+import os
+
 import numpy as np
 
 import csparks as cs
@@ -42,3 +45,27 @@ def species_close(a: np.ndarray, b: np.ndarray, rtol: float):
     tol = rtol * np.maximum(np.abs(b), 1e-6 * scale)
     bad = np.abs(a - b) > tol
     return not bad.any(), float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-6 * max(scale, 1e-300))))
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load_geometry_fixture(name: str) -> cs.Geometry:
+    """A reference geometry extracted by tests/golden/make_fixtures.py."""
+    meta = json.load(open(os.path.join(GOLDEN, name + ".json")))
+    z = np.load(os.path.join(GOLDEN, name + ".npz"))
+    bnds = [cs.Boundary(b["name"], b["type"], z["boundary_" + b["name"]]) for b in meta["boundaries"]]
+    return cs.Geometry(voxels=z["voxels"], h=meta["h"], domain_names=meta["domain_names"],
+                       domain_connections=z["domain_connections"], boundaries=bnds,
+                       global_size=tuple(meta["global_size"]))
+
+
+def load_model_fixture(name: str) -> cs.SpeciesModel:
+    d = json.load(open(os.path.join(GOLDEN, name + ".json")))
+    ch = lambda c: cs.ChannelModel(**c) if c else None
+    return cs.SpeciesModel(domain_names=d["domain_names"], species_names=d["species_names"],
+                           diffusive=d["diffusive"], sigma=d["sigma"], init=np.array(d["init"]),
+                           buffers=[[cs.Buffer(**b) for b in dom] for dom in d["buffers"]],
+                           ryr=ch(d["ryr"]), p_ryr=ch(d["p_ryr"]),
+                           serca=cs.SercaModel(**d["serca"]) if d["serca"] else None,
+                           conv=tuple(d["conv"]), fixed=[tuple(x) for x in d["fixed"]])

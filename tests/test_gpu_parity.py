@@ -140,6 +140,50 @@ def test_fp32_against_fp64_oracle():
     sim.close()
 
 
+LAUNCHES = [dict(rows=8, zchunk=3, xcd_remap=1, prefetch=2, nontemporal=1, lds_tile=0),
+            dict(rows=1, zchunk=1, xcd_remap=0, prefetch=1, nontemporal=0, lds_tile=0),
+            dict(rows=2, zchunk=5, xcd_remap=1, prefetch=1, nontemporal=1, lds_tile=0),
+            dict(rows=4, zchunk=64, xcd_remap=0, prefetch=2, nontemporal=0, lds_tile=0),
+            dict(rows=4, zchunk=4, xcd_remap=1, prefetch=1, nontemporal=1, lds_tile=1),
+            dict(rows=8, zchunk=64, xcd_remap=0, prefetch=1, nontemporal=0, lds_tile=1),
+            dict(rows=1, zchunk=2, xcd_remap=0, prefetch=1, nontemporal=0, lds_tile=1),
+            dict(rows=2, zchunk=7, xcd_remap=1, prefetch=1, nontemporal=1, lds_tile=1)]
+
+
+@pytest.mark.parametrize("cfg", LAUNCHES)
+def test_launch_shapes_agree(cfg):
+    """Every launch shape the autotuner may pick gives the same answer."""
+    d = with_serca(dyad.small_dyad(seed=12))
+    opts = cs.Options(seed=4, open={"ryr": [d.open_index]}, t_close={"ryr": 1.0})
+    _say(f"case: launch {cfg}")
+    orc = Oracle(d.geometry, d.model, opts)
+    sim = cs.Simulation(d.geometry, d.model, opts)
+    sim.set_launch_config(**cfg)
+    assert sim.launch_config() == {**cfg, "zchunk": min(cfg["zchunk"], sim.n[2])}
+    sim.step(60)
+    orc.step(60)
+    _compare_fields(sim, orc, RTOL_F64)
+    sim.close()
+
+
+def test_autotune_keeps_state():
+    d = dyad.small_dyad(seed=13)
+    opts = cs.Options(seed=4, open={"ryr": [d.open_index]}, t_close={"ryr": 1.0})
+    sim = cs.Simulation(d.geometry, d.model, opts)
+    sim.step(10)
+    before = [sim.species(s) for s in range(sim.ns)]
+    cfg, ms = sim.autotune(reps=1)
+    _say(f"autotune small: {cfg} {ms * 1e3:.1f} us")
+    for s in range(sim.ns):
+        np.testing.assert_array_equal(sim.species(s), before[s])
+    orc = Oracle(d.geometry, d.model, opts)
+    orc.step(10)
+    sim.step(30)
+    orc.step(30)
+    _compare_fields(sim, orc, RTOL_F64)
+    sim.close()
+
+
 def test_stats_match_oracle():
     d = dyad.small_dyad(seed=2)
     opts = cs.Options(seed=5, open={"ryr": [d.open_index]}, t_close={"ryr": 1.0})
