@@ -108,6 +108,8 @@ struct LaunchCfg {
   int pf;        /* planes loaded ahead of the one being computed (1 or 2)  */
   int nt;        /* 1: non-temporal stores of the next state               */
   int lds;       /* 1: x/y neighbours from an LDS plane tile                */
+  int lin;       /* 1: waves take 64 consecutive (x,y) columns of the plane,
+                    wrapping across rows, instead of 64 x rows tiles       */
 };
 
 template <typename T>
@@ -121,7 +123,7 @@ template <typename T, int NS, int PF, bool NT>
 __global__ void __launch_bounds__(BX * 8)
 k_stencil(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__ ids,
           const DevParams<T>* __restrict__ P, Geo g, uint32_t dmask, uint32_t react,
-          int zchunk, int gx, int gy, int nblocks, int xcd)
+          int zchunk, int gx, int gy, int nblocks, int xcd, int lin)
 {
   __shared__ T s_coef[NS][MAXD * MAXD];
   __shared__ T s_kon[MAXS][MAXD], s_koff[MAXS][MAXD], s_tot[MAXS][MAXD];
@@ -155,10 +157,19 @@ k_stencil(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict
   if (tid < MAXD) s_fixed[tid] = P->fixed_mask[tid];
   __syncthreads();
 
-  const int x = bx * BX + threadIdx.x;
-  const int y = by * (int)blockDim.y + threadIdx.y;
-  if (x >= g.nx || y >= g.ny) return;
-  const int z0 = bz * zchunk;
+  int x, y;
+  if (lin) {
+    /* linear: gx counts blocks of 64*rows consecutive plane columns */
+    const int64_t c = ((int64_t)(bid % gx) * (int)blockDim.y + threadIdx.y) * BX + threadIdx.x;
+    if (c >= (int64_t)g.plane) return;
+    y = (int)(c / g.nx);
+    x = (int)(c - (int64_t)y * g.nx);
+  } else {
+    x = bx * BX + threadIdx.x;
+    y = by * (int)blockDim.y + threadIdx.y;
+    if (x >= g.nx || y >= g.ny) return;
+  }
+  const int z0 = (lin ? bid / gx : bz) * zchunk;
   const int z1 = min(g.nz, z0 + zchunk);
   if (z0 >= z1) return;
 
@@ -677,7 +688,7 @@ struct cs_context {
   std::vector<uint8_t> init_states;
   uint32_t substeps[MAXS] = {0};
   uint32_t flux_substep[MAXF] = {0};
-  LaunchCfg launch{4, 32, 1, 1, 0, 0};
+  LaunchCfg launch{4, 32, 1, 1, 0, 0, 0};
 };
 
 namespace {
@@ -695,7 +706,7 @@ constexpr size_t LDS_TILE_LIMIT = 60 * 1024;
  * times over, z chunks no shorter than a few planes. */
 LaunchCfg default_launch(const Geo& g)
 {
-  LaunchCfg L{4, 32, 1, 1, 0, 0};
+  LaunchCfg L{4, 32, 1, 1, 0, 0, 0};
   const uint64_t xy = (uint64_t)((g.nx + BX - 1) / BX) * ((g.ny + L.rows - 1) / L.rows);
   int chunks = (int)std::max<uint64_t>(1, (4096 + xy - 1) / xy);
   L.zchunk = std::max(4, (g.nz + chunks - 1) / chunks);
@@ -849,8 +860,16 @@ void launch_stencil_v(cs_context* c, const LaunchCfg& L, const void* uin, void* 
                       uint32_t react)
 {
   const Geo& g = c->geo;
-  const int gx = (g.nx + BX - 1) / BX, gy = (g.ny + L.rows - 1) / L.rows;
+  const bool lin = L.lin && !L.lds;
   const int gz = (g.nz + L.zchunk - 1) / L.zchunk;
+  int gx, gy;
+  if (lin) {
+    gx = (int)((g.plane + (uint64_t)BX * L.rows - 1) / ((uint64_t)BX * L.rows));
+    gy = 1;
+  } else {
+    gx = (g.nx + BX - 1) / BX;
+    gy = (g.ny + L.rows - 1) / L.rows;
+  }
   const int nblocks = gx * gy * gz;
   const int launch = L.xcd ? ((nblocks + 7) / 8) * 8 : nblocks;
   if (L.lds) {
@@ -861,7 +880,7 @@ void launch_stencil_v(cs_context* c, const LaunchCfg& L, const void* uin, void* 
   } else {
     hipLaunchKernelGGL((k_stencil<T, NS, PF, NT>), dim3(launch), dim3(BX, L.rows), 0, c->stream,
                        (const T*)uin, (T*)uout, c->d_ids, (const DevParams<T>*)c->d_params, g, dmask,
-                       react, L.zchunk, gx, gy, nblocks, L.xcd);
+                       react, L.zchunk, gx, gy, nblocks, L.xcd, lin ? 1 : 0);
   }
 }
 
@@ -1330,6 +1349,7 @@ int cs_get_launch_config(const cs_context* c, cs_launch_config* out)
   out->prefetch = (uint32_t)c->launch.pf;
   out->nontemporal = (uint32_t)c->launch.nt;
   out->lds_tile = (uint32_t)c->launch.lds;
+  out->linear = (uint32_t)c->launch.lin;
   return CS_OK;
 }
 
@@ -1340,7 +1360,9 @@ int cs_set_launch_config(cs_context* c, const cs_launch_config* in)
     return set_err(CS_EINVAL, "rows must be 1, 2, 4 or 8");
   if (in->zchunk < 1) return set_err(CS_EINVAL, "zchunk must be >= 1");
   if (in->prefetch != 1 && in->prefetch != 2) return set_err(CS_EINVAL, "prefetch must be 1 or 2");
-  if (in->xcd_remap > 1 || in->nontemporal > 1 || in->lds_tile > 1) return set_err(CS_EINVAL, "flags must be 0 or 1");
+  if (in->xcd_remap > 1 || in->nontemporal > 1 || in->lds_tile > 1 || in->linear > 1)
+    return set_err(CS_EINVAL, "flags must be 0 or 1");
+  if (in->lds_tile && in->linear) return set_err(CS_EINVAL, "the LDS tile kernel uses 2-D tiles");
   if (in->lds_tile && in->prefetch != 1) return set_err(CS_EINVAL, "the LDS tile kernel prefetches one plane");
   if (in->lds_tile && lds_tile_bytes(c->esize, c->ns, (int)in->rows) > LDS_TILE_LIMIT)
     return set_err(CS_EINVAL, "LDS tile of %u rows too large for %u species", in->rows, c->ns);
@@ -1351,6 +1373,7 @@ int cs_set_launch_config(cs_context* c, const cs_launch_config* in)
   L.pf = (int)in->prefetch;
   L.nt = (int)in->nontemporal;
   L.lds = (int)in->lds_tile;
+  L.lin = (int)in->linear;
   c->launch = L;
   return CS_OK;
 }
@@ -1390,15 +1413,16 @@ int cs_autotune(cs_context* c, uint64_t first_time_ind, uint32_t reps, cs_launch
     if (zc < nz) zcs.push_back(zc);
   zcs.push_back(nz);
   int rc = CS_OK;
-  for (int lds : {0, 1})
+  for (int kind : {0, 1, 2})   /* 2-D tiles, linear columns, LDS tiles */
   for (int rows : {2, 4, 8})
     for (int zc : zcs)
       for (int xcd : {0, 1})
         for (int pf : {1, 2})
           for (int nt : {0, 1}) {
+            const int lds = kind == 2, lin = kind == 1;
             if (lds && pf != 1) continue;
             if (lds && lds_tile_bytes(c->esize, c->ns, rows) > LDS_TILE_LIMIT) continue;
-            LaunchCfg L{rows, zc, xcd, pf, nt, lds};
+            LaunchCfg L{rows, zc, xcd, pf, nt, lds, lin};
             /* reads the current state, writes the other buffer (scratch
              * until the next step overwrites it); the state is unchanged */
             auto go = [&](uint32_t dm, uint32_t re) {

@@ -161,10 +161,12 @@ int cs_time_steps(cs_context* ctx, uint64_t first_time_ind, uint64_t nsteps,
  * XCDs works on a contiguous slab (shared L2 for halo rows); prefetch is
  * the number of planes loaded ahead (1 or 2); nontemporal streams the next
  * state past the caches; lds_tile stages each plane in LDS so the x/y
- * neighbours are read from LDS instead of memory.  No reference counterpart: the reference's
+ * neighbours are read from LDS instead of memory; linear hands each
+ * wavefront 64 consecutive (x,y) columns of the plane across row ends, so
+ * no lane idles when nx is not a multiple of 64.  No reference counterpart: the reference's
  * OpenMP/AVX-512 loop has no launch shape (kernel_generic.cpp:561-600). */
 typedef struct cs_launch_config {
-  uint32_t rows, zchunk, xcd_remap, prefetch, nontemporal, lds_tile;
+  uint32_t rows, zchunk, xcd_remap, prefetch, nontemporal, lds_tile, linear;
 } cs_launch_config;
 
 int cs_get_launch_config(const cs_context* ctx, cs_launch_config* cfg);

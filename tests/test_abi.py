@@ -1,0 +1,49 @@
+"""The C-ABI libraries load and export every function their headers
+declare (CPU only: nothing is called that needs a GPU)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+import csparks as cs
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = {
+    "csparks.h": "libcsparks.so",
+    "cs_host.h": "libcshost.so",
+    "cs_io.h": "libcsio.so",
+    "cs_model.h": "libcshost.so",
+}
+
+
+def declared_functions(header):
+    text = open(os.path.join(REPO, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"typedef struct[^;]*?\{.*?\}[^;]*;", "", text, flags=re.S)
+    names = re.findall(r"^[A-Za-z_][\w\s\*]*?\b(cs_\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+@pytest.mark.parametrize("header,lib", sorted(HEADERS.items()))
+def test_library_exports_every_declared_function(header, lib):
+    names = declared_functions(header)
+    assert names, header
+    dll = C.CDLL(os.path.join(cs.BUILD, lib))
+    missing = [n for n in names if not hasattr(dll, n)]
+    assert not missing, f"{lib} lacks {missing}"
+
+
+def test_host_struct_layouts_match_python_mirror():
+    h = cs.host_lib()
+    for i, st in enumerate([cs.CGeometry, cs.CSpeciesModel, cs.COptions, cs.CProblem, cs.CSetup, cs.CFluxDesc]):
+        assert h.cs_host_sizeof(i) == C.sizeof(st), st.__name__
+
+
+def test_error_paths_without_gpu():
+    """Argument checks fire before any device call."""
+    lib = cs.device_lib()
+    ctx = C.c_void_p()
+    assert lib.cs_create(None, C.byref(ctx)) == -1
+    assert b"NULL" in lib.cs_last_error()
+    assert lib.cs_step(None, 1, 1) == -1
