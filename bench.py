@@ -73,7 +73,7 @@ def main():
     ap.add_argument("--size", type=int, nargs=3, default=[400, 400, 200])
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--launch", type=str, default="",
-                    help="fixed launch shape rows,zchunk,xcd_remap,prefetch,nontemporal,lds_tile (skips autotune)")
+                    help="fixed launch shape rows,zchunk,xcd_remap,prefetch,nontemporal,lds_tile,linear,yblock (skips autotune)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB (for scaling traffic counters) and exit")
     args = ap.parse_args()
@@ -99,7 +99,7 @@ def main():
     sim.synchronize()
     launch = sim.launch_config()
     if args.launch:
-        keys = ["rows", "zchunk", "xcd_remap", "prefetch", "nontemporal", "lds_tile"]
+        keys = ["rows", "zchunk", "xcd_remap", "prefetch", "nontemporal", "lds_tile", "linear", "yblock"]
         sim.set_launch_config(**dict(zip(keys, (int(v) for v in args.launch.split(",")))))
         launch = sim.launch_config()
     elif not args.no_autotune:

@@ -173,7 +173,7 @@ class CSetup(C.Structure):
 class CLaunchConfig(C.Structure):
     _fields_ = [("rows", C.c_uint32), ("zchunk", C.c_uint32), ("xcd_remap", C.c_uint32),
                 ("prefetch", C.c_uint32), ("nontemporal", C.c_uint32), ("lds_tile", C.c_uint32),
-                ("linear", C.c_uint32)]
+                ("linear", C.c_uint32), ("yblock", C.c_uint32)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
@@ -248,6 +248,8 @@ def device_lib() -> C.CDLL:
         lib.cs_uniforms_needed.argtypes = [P, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64)]
         lib.cs_synchronize.argtypes = [P]
         lib.cs_reduce_stats.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        lib.cs_linescan.argtypes = [P, C.c_uint32, C.c_uint32] + [C.POINTER(C.c_double)] * 3 + \
+            [C.c_uint32, C.POINTER(C.c_double)]
         lib.cs_get_channel_states.argtypes = [P, C.c_uint32, C.POINTER(C.c_uint8)]
         lib.cs_set_channel_states.argtypes = [P, C.c_uint32, C.POINTER(C.c_uint8)]
         lib.cs_time_steps.argtypes = [P, C.c_uint64, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -367,6 +369,11 @@ class SpeciesModel:
     serca: Optional[SercaModel] = None
     conv: Tuple[float, float, float] = (28853.9, 28853.9, 115416.0)
     fixed: List[Tuple[int, int]] = field(default_factory=list)
+
+
+def gauss(x, sigma):
+    """Gaussian weight of the linescan (gauss, utils.c:209-213)."""
+    return np.sqrt(0.5) * (2.0 / np.sqrt(np.pi)) * (1.0 / np.sqrt(sigma)) / 2 * np.exp(-np.asarray(x) * x / 2 / sigma)
 
 
 @dataclass
@@ -738,6 +745,23 @@ class Simulation:
         ptr = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
         _check(device_lib().cs_reduce_stats(self.ctx, ptr(mn), ptr(mx), ptr(tot)), "cs_reduce_stats")
         return mn.reshape(self.nd, self.ns), mx.reshape(self.nd, self.ns), tot.reshape(self.nd, self.ns)
+
+    def linescan(self, species: int, axis: int, offsets, conv, num_domains: int = 0) -> np.ndarray:
+        """Linescan of one species along a reference axis, as written by
+        Species_write_convolution_to_file (species.c:1960-2102): the
+        transverse Gaussian weights project the field on the device
+        (cs_linescan), and the projection is convolved along the axis."""
+        n = self.n
+        h = self.setup.problem.h
+        off = [(n[ax] // 2 + int(offsets[ax])) * h for ax in range(3)]
+        w = [np.ascontiguousarray(gauss(off[ax] - np.arange(n[ax]) * h, conv[ax])) for ax in range(3)]
+        mask = 0xFFFFFFFF if not num_domains else (0xFF if num_domains >= 8 else (1 << num_domains) - 1)
+        proj = np.empty(n[axis])
+        ptr = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+        _check(device_lib().cs_linescan(self.ctx, species, axis, ptr(w[0]), ptr(w[1]), ptr(w[2]), mask, ptr(proj)),
+               "cs_linescan")
+        k = np.arange(n[axis]) * h
+        return np.array([float(np.dot(proj, gauss(p - k, conv[axis]))) for p in k])
 
     def channel_states(self, flux: int) -> np.ndarray:
         n = self.setup.problem.fluxes[flux].num_pairs

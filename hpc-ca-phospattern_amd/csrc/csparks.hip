@@ -78,6 +78,13 @@ struct FluxParams {
   double   p[CS_NUM_RYR_PARAMS]; /* reference parameter vector           */
 };
 
+/* Rates of the buffer that binds a species in one domain (zero when none
+ * does); padded to 4 values so one LDS read fetches them. */
+template <typename T>
+struct RateTab {
+  T kon, koff, tot, pad;
+};
+
 template <typename T>
 struct DevParams {
   T coef[MAXS][MAXD * MAXD];
@@ -89,6 +96,12 @@ struct DevParams {
   uint32_t fixed_mask[MAXD];
   uint32_t dirichlet;
   uint32_t nflux, ndiscrete;
+  /* species-major form of the buffer reactions: rt[d][s] holds the rates of
+   * the buffer binding species s in domain d; rbound marks the species some
+   * buffer binds; rfast is set when every buffer of every domain binds
+   * species 0 and no species is bound by two buffers of one domain */
+  RateTab<T> rt[MAXD][MAXS];
+  uint32_t rbound, rfast;
   double dt, h;
   FluxParams flux[MAXF];
 };
@@ -110,7 +123,92 @@ struct LaunchCfg {
   int lds;       /* 1: x/y neighbours from an LDS plane tile                */
   int lin;       /* 1: waves take 64 consecutive (x,y) columns of the plane,
                     wrapping across rows, instead of 64 x rows tiles       */
+  int yb;        /* >0: row-blocked kernel, yb rows per lane and `rows`
+                    wavefronts side by side along x per block              */
 };
+
+/* Per-block copies of the tables every voxel update reads. */
+template <typename T, int NS>
+struct BlockTables {
+  T coef[NS][MAXD * MAXD];
+  T kon[MAXS][MAXD], koff[MAXS][MAXD], tot[MAXS][MAXD];
+  T init[MAXD][NS];
+  RateTab<T> rt[MAXD][NS];
+  uint8_t b0[MAXS][MAXD], b1[MAXS][MAXD];
+  uint32_t fixed[MAXD];
+
+  __device__ void load(const DevParams<T>* __restrict__ P, int tid, int nthr)
+  {
+    for (int i = tid; i < NS * MAXD * MAXD; i += nthr) coef[i / (MAXD * MAXD)][i % (MAXD * MAXD)] = P->coef[i / (MAXD * MAXD)][i % (MAXD * MAXD)];
+    for (int i = tid; i < MAXS * MAXD; i += nthr) {
+      const int b = i / MAXD, d = i % MAXD;
+      kon[b][d] = P->kon[b][d];
+      koff[b][d] = P->koff[b][d];
+      tot[b][d] = P->tot[b][d];
+      b0[b][d] = P->bsp0[b][d];
+      b1[b][d] = P->bsp1[b][d];
+    }
+    for (int i = tid; i < MAXD * NS; i += nthr) {
+      init[i / NS][i % NS] = P->init[i / NS][i % NS];
+      rt[i / NS][i % NS] = P->rt[i / NS][i % NS];
+    }
+    if (tid < MAXD) fixed[tid] = P->fixed_mask[tid];
+  }
+};
+
+/* Buffer reactions in the reference's order (reaction_SIMD,
+ * kernel_generic.cpp:397-509, called per buffer from species.c:1689-1692):
+ * for every buffer slot b of the voxel's domain, J = k_on (tot - B) F -
+ * k_off B, then B += dt J and F -= dt J, where a species that did not
+ * diffuse this step is overwritten from u instead of accumulated.  Species
+ * indices vary per slot and domain, so they are selected per lane. */
+template <typename T, int NS>
+__device__ inline void react_general(T nw[NS], const T uc[NS], const BlockTables<T, NS>& S, int d, int nb,
+                                     uint32_t dmask, T dtr)
+{
+  for (int b = 0; b < nb; b++) {
+    const int b0 = S.b0[b][d], b1 = S.b1[b][d];
+    T B0 = uc[0], B1 = uc[0];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      if (s == b0) B0 = uc[s];
+      if (s == b1) B1 = uc[s];
+    }
+    const T J = S.kon[b][d] * (S.tot[b][d] - B0) * B1 - S.koff[b][d] * B0;
+    const T inc = dtr * J;
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+      if (s == b0) nw[s] = (((dmask >> s) & 1u) ? nw[s] : uc[s]) + inc;
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+      if (s == b1) nw[s] = (((dmask >> s) & 1u) ? nw[s] : uc[s]) - inc;
+  }
+}
+
+/* The same reactions in species-major form, for models in which every
+ * buffer binds species 0 and each bound species belongs to one buffer of
+ * its domain, on steps where species 0 diffuses: then no overwrite can
+ * discard an increment (a bound species is written once, and nw = uc for
+ * one that did not diffuse), slots a domain lacks are exact no-ops, and
+ * the result equals react_general's up to the order in which species 0
+ * sums its increments (species order here, slot order there).  Species
+ * indices are compile-time, so no per-lane selection is needed. */
+template <typename T, int NS>
+__device__ inline void react_species_major(T nw[NS], const T uc[NS], const BlockTables<T, NS>& S, int d,
+                                           uint32_t rbound, T dtr)
+{
+  T f = nw[0];
+#pragma unroll
+  for (int s = 1; s < NS; s++) {
+    if (!((rbound >> s) & 1u)) continue;
+    const RateTab<T> r = S.rt[d][s];
+    const T J = r.kon * (r.tot - uc[s]) * uc[0] - r.koff * uc[s];
+    const T inc = dtr * J;
+    nw[s] += inc;
+    f -= inc;
+  }
+  nw[0] = f;
+}
 
 template <typename T>
 __device__ inline void store_next(T* p, T v, bool nt)
@@ -119,17 +217,88 @@ __device__ inline void store_next(T* p, T v, bool nt)
   else *p = v;
 }
 
+/* A wavefront-uniform pointer, passed through readfirstlane so that loop
+ * optimisations cannot fold it into per-lane 64-bit address arithmetic. */
+/* Loads and stores at a wavefront-uniform base plus a per-lane 32-bit
+ * byte offset (the global_load/store saddr form: no per-lane address
+ * arithmetic).  Inside loops the offsets go through pin_offsets() first:
+ * otherwise the compiler hoists their zero extension out of the loop and
+ * falls back to 64-bit per-lane addresses. */
+template <typename T>
+__device__ inline T load_at(const T* base, uint32_t byte_off)
+{
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
+template <typename T>
+__device__ inline void store_at(T* base, uint32_t byte_off, T v, bool nt)
+{
+  store_next(reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off), v, nt);
+}
+
+__device__ inline void pin_offset(uint32_t& v) { asm volatile("" : "+v"(v)); }
+
+template <typename... U>
+__device__ inline void pin_offsets(U&... v)
+{
+  (pin_offset(v), ...);
+}
+
+/* Raw buffer access: a descriptor (base and byte size, uniform across the
+ * wavefront), a per-lane byte offset and a uniform byte offset, e.g. the
+ * plane.  Nothing is left for the compiler to turn into per-lane 64-bit
+ * address arithmetic, and an access beyond the descriptor's size reads 0
+ * and writes nothing instead of faulting.  Word 3 is the raw-buffer
+ * format word of the gfx9 family (32-bit data format). */
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+
+__device__ inline Rsrc make_rsrc(const void* base, uint32_t bytes)
+{
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
+}
+
+template <typename T>
+__device__ inline T bload(Rsrc r, uint32_t voff, uint32_t soff);
+template <>
+__device__ inline float bload<float>(Rsrc r, uint32_t voff, uint32_t soff)
+{
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+template <>
+__device__ inline double bload<double>(Rsrc r, uint32_t voff, uint32_t soff)
+{
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+__device__ inline uint32_t bload_u8(Rsrc r, uint32_t voff, uint32_t soff)
+{
+  return __builtin_amdgcn_raw_buffer_load_b8(r, (int)voff, (int)soff, 0);
+}
+
+/* aux 2 sets the non-temporal bit (nt) */
+template <typename T>
+__device__ inline void bstore(Rsrc r, uint32_t voff, uint32_t soff, T v, bool nt);
+template <>
+__device__ inline void bstore<float>(Rsrc r, uint32_t voff, uint32_t soff, float v, bool nt)
+{
+  if (nt) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff, (int)soff, 2);
+  else __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff, (int)soff, 0);
+}
+template <>
+__device__ inline void bstore<double>(Rsrc r, uint32_t voff, uint32_t soff, double v, bool nt)
+{
+  typedef unsigned int u32x2 __attribute__((__vector_size__(2 * sizeof(unsigned int))));
+  const u32x2 b = __builtin_bit_cast(u32x2, v);
+  if (nt) __builtin_amdgcn_raw_buffer_store_b64(b, r, (int)voff, (int)soff, 2);
+  else __builtin_amdgcn_raw_buffer_store_b64(b, r, (int)voff, (int)soff, 0);
+}
+
 template <typename T, int NS, int PF, bool NT>
 __global__ void __launch_bounds__(BX * 8)
 k_stencil(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__ ids,
           const DevParams<T>* __restrict__ P, Geo g, uint32_t dmask, uint32_t react,
           int zchunk, int gx, int gy, int nblocks, int xcd, int lin)
 {
-  __shared__ T s_coef[NS][MAXD * MAXD];
-  __shared__ T s_kon[MAXS][MAXD], s_koff[MAXS][MAXD], s_tot[MAXS][MAXD];
-  __shared__ T s_init[MAXD][NS];
-  __shared__ uint8_t s_b0[MAXS][MAXD], s_b1[MAXS][MAXD];
-  __shared__ uint32_t s_fixed[MAXD];
+  __shared__ BlockTables<T, NS> S;
 
   /* tile of this block; with xcd set, the hardware's round-robin of blocks
    * over the 8 XCDs hands each XCD a contiguous run of tiles, so halo rows
@@ -144,17 +313,7 @@ k_stencil(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict
 
   const int nthr = BX * blockDim.y;
   const int tid = threadIdx.y * BX + threadIdx.x;
-  for (int i = tid; i < NS * MAXD * MAXD; i += nthr) s_coef[i / (MAXD * MAXD)][i % (MAXD * MAXD)] = P->coef[i / (MAXD * MAXD)][i % (MAXD * MAXD)];
-  for (int i = tid; i < MAXS * MAXD; i += nthr) {
-    const int b = i / MAXD, d = i % MAXD;
-    s_kon[b][d] = P->kon[b][d];
-    s_koff[b][d] = P->koff[b][d];
-    s_tot[b][d] = P->tot[b][d];
-    s_b0[b][d] = P->bsp0[b][d];
-    s_b1[b][d] = P->bsp1[b][d];
-  }
-  for (int i = tid; i < MAXD * NS; i += nthr) s_init[i / NS][i % NS] = P->init[i / NS][i % NS];
-  if (tid < MAXD) s_fixed[tid] = P->fixed_mask[tid];
+  S.load(P, tid, nthr);
   __syncthreads();
 
   int x, y;
@@ -173,65 +332,105 @@ k_stencil(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict
   const int z1 = min(g.nz, z0 + zchunk);
   if (z0 >= z1) return;
 
-  const int nb = react ? (int)P->nb : 0;
+  const int nb = react == 1 ? (int)P->nb : 0;
+  const uint32_t rbound = P->rbound;
   const T dtr = P->dt_react;
   const bool dirichlet = P->dirichlet != 0;
   /* clamped neighbour offsets: a missing neighbour reads the voxel
-   * itself, so its (u_n - u) term vanishes -- the no-flux outer faces */
-  const int64_t oxm = x > 0 ? -1 : 0, oxp = x + 1 < g.nx ? 1 : 0;
-  const int64_t oym = y > 0 ? -(int64_t)g.nx : 0, oyp = y + 1 < g.ny ? (int64_t)g.nx : 0;
+   * itself, so its (u_n - u) term vanishes -- the no-flux outer faces.
+   * A lane's column is fixed while it marches z, so its offsets within a
+   * plane are computed once, as 32-bit byte offsets from a per-plane,
+   * per-species base address that is uniform across the wavefront (the
+   * host checks plane * sizeof(T) < 2^31); the loop does no per-lane
+   * address arithmetic. */
+  const uint32_t ES = (uint32_t)sizeof(T);
+  const uint32_t col = (uint32_t)y * (uint32_t)g.nx + (uint32_t)x;
+  const uint32_t bc = col * ES;
+  const uint32_t bxm = (x > 0 ? col - 1 : col) * ES, bxp = (x + 1 < g.nx ? col + 1 : col) * ES;
+  const uint32_t bym = (y > 0 ? col - g.nx : col) * ES, byp = (y + 1 < g.ny ? col + g.nx : col) * ES;
   const bool xyface = x == 0 || y == 0 || x + 1 == g.nx || y + 1 == g.ny;
-  const uint64_t col = (uint64_t)y * g.nx + x;
   const int ztop_plane = g.nz - 1;
+  const size_t plane = g.plane;
+
+  /* descriptors over the planes this block touches: [zb, ze) of every
+   * species of u and of the ids, [z0, z1) of the next state */
+  const int zb = max(z0 - 1, 0), ze = min(g.nz, z1 + 2);
+  const uint32_t pbytes = (uint32_t)plane * ES;
+  Rsrc rin[NS], rout[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    rin[s] = make_rsrc(u + s * g.nv + (size_t)zb * plane, (uint32_t)(ze - zb) * pbytes);
+    rout[s] = make_rsrc(un + s * g.nv + (size_t)z0 * plane, (uint32_t)(z1 - z0) * pbytes);
+  }
+  const Rsrc rid = make_rsrc(ids + (size_t)zb * plane, (uint32_t)(ze - zb) * (uint32_t)plane);
 
   /* register ring over z: um = z-1, uc = z, up = z+1, upp = z+2 (PF 2);
    * plane indices are clamped, so a missing z neighbour equals uc */
   T um[NS], uc[NS], up[NS], upp[NS];
-  {
-    const uint64_t vc = (uint64_t)z0 * g.plane + col;
-    const uint64_t vm = (uint64_t)max(z0 - 1, 0) * g.plane + col;
-    const uint64_t vp = (uint64_t)min(z0 + 1, ztop_plane) * g.plane + col;
 #pragma unroll
-    for (int s = 0; s < NS; s++) {
-      uc[s] = u[s * g.nv + vc];
-      um[s] = u[s * g.nv + vm];
-      if (PF == 2) up[s] = u[s * g.nv + vp];
-    }
+  for (int s = 0; s < NS; s++) {
+    uc[s] = bload<T>(rin[s], bc, (uint32_t)(z0 - zb) * pbytes);
+    um[s] = bload<T>(rin[s], bc, (uint32_t)(max(z0 - 1, 0) - zb) * pbytes);
+    if (PF == 2) up[s] = bload<T>(rin[s], bc, (uint32_t)(min(z0 + 1, ztop_plane) - zb) * pbytes);
   }
+  /* drain the prologue loads, so the loop header's wait does not also
+   * cover the previous plane's stores (stores count in vmcnt on gfx9) */
+  __builtin_amdgcn_s_waitcnt(0);
   for (int z = z0; z < z1; z++) {
-    const uint64_t v = (uint64_t)z * g.plane + col;
+    const uint32_t sz = (uint32_t)(z - zb) * pbytes;           /* plane z in rin */
+    const uint32_t szi = (uint32_t)(z - zb) * (uint32_t)plane; /* plane z in rid */
     const bool ztop = z < ztop_plane;
     if (PF == 1) {
-      const uint64_t vp = (uint64_t)min(z + 1, ztop_plane) * g.plane + col;
+      const uint32_t sp = (uint32_t)(min(z + 1, ztop_plane) - zb) * pbytes;
 #pragma unroll
-      for (int s = 0; s < NS; s++) up[s] = u[s * g.nv + vp];
+      for (int s = 0; s < NS; s++) up[s] = bload<T>(rin[s], bc, sp);
     } else if (z + 2 <= z1) {
-      const uint64_t vpp = (uint64_t)min(z + 2, ztop_plane) * g.plane + col;
+      const uint32_t sq = (uint32_t)(min(z + 2, ztop_plane) - zb) * pbytes;
 #pragma unroll
-      for (int s = 0; s < NS; s++) upp[s] = u[s * g.nv + vpp];
+      for (int s = 0; s < NS; s++) upp[s] = bload<T>(rin[s], bc, sq);
     }
-
-    const uint8_t id = ids[v];
+    /* the in-plane neighbours of every diffusing species are loaded before
+     * any of them is used, so one wait covers them all */
+    const uint32_t id = bload_u8(rid, col, szi);
+    T nxm[NS], nxp[NS], nym[NS], nyp[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      nxm[s] = nxp[s] = nym[s] = nyp[s] = uc[s];
+      if ((dmask >> s) & 1u) {
+        nxm[s] = bload<T>(rin[s], bxm, sz);
+        nxp[s] = bload<T>(rin[s], bxp, sz);
+        nym[s] = bload<T>(rin[s], bym, sz);
+        nyp[s] = bload<T>(rin[s], byp, sz);
+      }
+    }
+    /* with one plane ahead, the next plane is complete before the first
+     * store, so only stores are outstanding at the loop back edge and the
+     * ring copies below need no wait there */
+    if (PF == 1) {
+#pragma unroll
+      for (int s = 0; s < NS; s++) asm volatile("" ::"v"(up[s]));
+    }
     const int d = id & 7;
     const bool uni = (id & 0x80) != 0;
     int dxm = d, dxp = d, dym = d, dyp = d, dzm = d, dzp = d;
     if (!uni) {
-      dxm = ids[v + oxm] & 7; dxp = ids[v + oxp] & 7;
-      dym = ids[v + oym] & 7; dyp = ids[v + oyp] & 7;
-      dzm = z > 0 ? ids[v - g.plane] & 7 : d;
-      dzp = ztop ? ids[v + g.plane] & 7 : d;
+      dxm = bload_u8(rid, bxm / ES, szi) & 7;
+      dxp = bload_u8(rid, bxp / ES, szi) & 7;
+      dym = bload_u8(rid, bym / ES, szi) & 7;
+      dyp = bload_u8(rid, byp / ES, szi) & 7;
+      dzm = z > 0 ? bload_u8(rid, col, szi - (uint32_t)plane) & 7 : d;
+      dzp = ztop ? bload_u8(rid, col, szi + (uint32_t)plane) & 7 : d;
     }
     T nw[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) {
       T acc = uc[s];
       if ((dmask >> s) & 1u) {
-        const T* us = u + s * g.nv;
         const T c = uc[s];
-        const T exm = us[v + oxm] - c, exp_ = us[v + oxp] - c;
-        const T eym = us[v + oym] - c, eyp = us[v + oyp] - c;
+        const T exm = nxm[s] - c, exp_ = nxp[s] - c;
+        const T eym = nym[s] - c, eyp = nyp[s] - c;
         const T ezm = um[s] - c, ezp = up[s] - c;
-        const T* C = s_coef[s] + d * MAXD;
+        const T* C = S.coef[s] + d * MAXD;
         if (uni) {
           acc = c + C[d] * ((ezm + ezp) + (eym + eyp) + (exm + exp_));
         } else {
@@ -241,36 +440,19 @@ k_stencil(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict
       }
       nw[s] = acc;
     }
-    /* buffers in the reference's order: the bound species first, then the
-     * free one; a species not updated earlier this step is overwritten
-     * from u rather than accumulated, as reaction_SIMD does */
-    for (int b = 0; b < nb; b++) {
-      const int b0 = s_b0[b][d], b1 = s_b1[b][d];
-      T B0 = uc[0], B1 = uc[0];
-#pragma unroll
-      for (int s = 0; s < NS; s++) {
-        if (s == b0) B0 = uc[s];
-        if (s == b1) B1 = uc[s];
-      }
-      const T J = s_kon[b][d] * (s_tot[b][d] - B0) * B1 - s_koff[b][d] * B0;
-      const T inc = dtr * J;
-#pragma unroll
-      for (int s = 0; s < NS; s++)
-        if (s == b0) nw[s] = (((dmask >> s) & 1u) ? nw[s] : uc[s]) + inc;
-#pragma unroll
-      for (int s = 0; s < NS; s++)
-        if (s == b1) nw[s] = (((dmask >> s) & 1u) ? nw[s] : uc[s]) - inc;
-    }
-    if (dirichlet && (xyface || z == 0 || !ztop)) nw[0] = s_init[d][0];
-    const uint32_t fm = s_fixed[d];
+    if (react == 2) react_species_major<T, NS>(nw, uc, S, d, rbound, dtr);
+    else react_general<T, NS>(nw, uc, S, d, nb, dmask, dtr);
+    if (dirichlet && (xyface || z == 0 || !ztop)) nw[0] = S.init[d][0];
+    const uint32_t fm = S.fixed[d];
     if (fm) {
 #pragma unroll
       for (int s = 0; s < NS; s++)
-        if ((fm >> s) & 1u) nw[s] = s_init[d][s];
+        if ((fm >> s) & 1u) nw[s] = S.init[d][s];
     }
+    const uint32_t so = (uint32_t)(z - z0) * pbytes;
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-      store_next(un + s * g.nv + v, nw[s], NT);
+      bstore<T>(rout[s], bc, so, nw[s], NT);
       um[s] = uc[s];
       uc[s] = up[s];
       if (PF == 2) up[s] = upp[s];
@@ -293,11 +475,7 @@ k_stencil_lds(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __rest
 {
   constexpr int TW = BX + 2;
   extern __shared__ unsigned char lds_raw[];
-  __shared__ T s_coef[NS][MAXD * MAXD];
-  __shared__ T s_kon[MAXS][MAXD], s_koff[MAXS][MAXD], s_tot[MAXS][MAXD];
-  __shared__ T s_init[MAXD][NS];
-  __shared__ uint8_t s_b0[MAXS][MAXD], s_b1[MAXS][MAXD];
-  __shared__ uint32_t s_fixed[MAXD];
+  __shared__ BlockTables<T, NS> S;
 
   int bid = blockIdx.x;
   if (xcd) {
@@ -309,17 +487,7 @@ k_stencil_lds(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __rest
   const int bx = bid % gx, by = (bid / gx) % gy, bz = bid / (gx * gy);
   const int nthr = BX * rows;
   const int tid = threadIdx.y * BX + threadIdx.x;
-  for (int i = tid; i < NS * MAXD * MAXD; i += nthr) s_coef[i / (MAXD * MAXD)][i % (MAXD * MAXD)] = P->coef[i / (MAXD * MAXD)][i % (MAXD * MAXD)];
-  for (int i = tid; i < MAXS * MAXD; i += nthr) {
-    const int b = i / MAXD, d = i % MAXD;
-    s_kon[b][d] = P->kon[b][d];
-    s_koff[b][d] = P->koff[b][d];
-    s_tot[b][d] = P->tot[b][d];
-    s_b0[b][d] = P->bsp0[b][d];
-    s_b1[b][d] = P->bsp1[b][d];
-  }
-  for (int i = tid; i < MAXD * NS; i += nthr) s_init[i / NS][i % NS] = P->init[i / NS][i % NS];
-  if (tid < MAXD) s_fixed[tid] = P->fixed_mask[tid];
+  S.load(P, tid, nthr);
 
   /* tile[buf][s][row 0..rows+1][col 0..65] */
   T* tile = reinterpret_cast<T*>(lds_raw);
@@ -349,7 +517,8 @@ k_stencil_lds(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __rest
   const uint64_t hy2col = (uint64_t)min(y + 1, g.ny - 1) * g.nx + xc;
   const int hy2slot = 2 * TW + tx + 1;
 
-  const int nb = react ? (int)P->nb : 0;
+  const int nb = react == 1 ? (int)P->nb : 0;
+  const uint32_t rbound = P->rbound;
   const T dtr = P->dt_react;
   const bool dirichlet = P->dirichlet != 0;
   const bool xyface = x == 0 || y == 0 || x + 1 == g.nx || y + 1 == g.ny;
@@ -419,7 +588,7 @@ k_stencil_lds(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __rest
           const T exm = ts[-1] - c, exp_ = ts[1] - c;
           const T eym = ts[-TW] - c, eyp = ts[TW] - c;
           const T ezm = um[s] - c, ezp = up[s] - c;
-          const T* C = s_coef[s] + d * MAXD;
+          const T* C = S.coef[s] + d * MAXD;
           if (uni) {
             acc = c + C[d] * ((ezm + ezp) + (eym + eyp) + (exm + exp_));
           } else {
@@ -429,29 +598,14 @@ k_stencil_lds(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __rest
         }
         nw[s] = acc;
       }
-      for (int b = 0; b < nb; b++) {
-        const int b0 = s_b0[b][d], b1 = s_b1[b][d];
-        T B0 = uc[0], B1 = uc[0];
-#pragma unroll
-        for (int s = 0; s < NS; s++) {
-          if (s == b0) B0 = uc[s];
-          if (s == b1) B1 = uc[s];
-        }
-        const T J = s_kon[b][d] * (s_tot[b][d] - B0) * B1 - s_koff[b][d] * B0;
-        const T inc = dtr * J;
-#pragma unroll
-        for (int s = 0; s < NS; s++)
-          if (s == b0) nw[s] = (((dmask >> s) & 1u) ? nw[s] : uc[s]) + inc;
-#pragma unroll
-        for (int s = 0; s < NS; s++)
-          if (s == b1) nw[s] = (((dmask >> s) & 1u) ? nw[s] : uc[s]) - inc;
-      }
-      if (dirichlet && (xyface || z == 0 || !ztop)) nw[0] = s_init[d][0];
-      const uint32_t fm = s_fixed[d];
+      if (react == 2) react_species_major<T, NS>(nw, uc, S, d, rbound, dtr);
+      else react_general<T, NS>(nw, uc, S, d, nb, dmask, dtr);
+      if (dirichlet && (xyface || z == 0 || !ztop)) nw[0] = S.init[d][0];
+      const uint32_t fm = S.fixed[d];
       if (fm) {
 #pragma unroll
         for (int s = 0; s < NS; s++)
-          if ((fm >> s) & 1u) nw[s] = s_init[d][s];
+          if ((fm >> s) & 1u) nw[s] = S.init[d][s];
       }
 #pragma unroll
       for (int s = 0; s < NS; s++) store_next(un + s * g.nv + v, nw[s], NT);
@@ -462,6 +616,245 @@ k_stencil_lds(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __rest
       uc[s] = up[s];
     }
     buf ^= 1;
+  }
+}
+
+/* Cross-lane moves within one 64-wide wavefront (DPP wave_shr:1 and
+ * wave_shl:1).  from_left: lane i receives lane i-1's value and lane 0
+ * keeps `edge`; from_right: lane i receives lane i+1's value and lane 63
+ * keeps `edge`.  All 64 lanes must be active. */
+__device__ inline float dpp_from_left(float v, float edge)
+{
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+__device__ inline float dpp_from_right(float v, float edge)
+{
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x130, 0xF, 0xF, false));
+}
+__device__ inline double dpp_from_left(double v, double edge)
+{
+  const uint64_t a = (uint64_t)__double_as_longlong(v), e = (uint64_t)__double_as_longlong(edge);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)e, (int)(uint32_t)a, 0x138, 0xF, 0xF, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)(e >> 32), (int)(uint32_t)(a >> 32), 0x138, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ inline double dpp_from_right(double v, double edge)
+{
+  const uint64_t a = (uint64_t)__double_as_longlong(v), e = (uint64_t)__double_as_longlong(edge);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)e, (int)(uint32_t)a, 0x130, 0xF, 0xF, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)(e >> 32), (int)(uint32_t)(a >> 32), 0x130, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+/* Row-blocked variant.  A wavefront owns 64 consecutive x of Y consecutive
+ * rows and marches z; a block is `rows` wavefronts side by side along x, so
+ * a block of ceil(nx/64) wavefronts covers whole rows.  Per plane and
+ * species each lane loads its Y own values (fully used 256-byte rows), the
+ * two rows bordering the Y rows (y halo) and, in lanes 0 and 63 only, the
+ * x neighbour beyond the wavefront; every other x neighbour comes from the
+ * adjacent lane through DPP and every y neighbour from the lane's own
+ * registers.  All loads of a plane are issued before the first wait.
+ * Coordinates are clamped (rows past ny, lanes past nx and planes past the
+ * grid repeat the last one), so a missing neighbour equals the voxel itself:
+ * the no-flux faces, with no per-face selects.  Byte offsets within one
+ * species plane are 32-bit (the host checks plane * sizeof(T) < 2^31). */
+template <typename T, int NS, int Y, int PF, bool NT>
+__global__ void __launch_bounds__(BX * 8)
+k_stencil_rows(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__ ids,
+               const DevParams<T>* __restrict__ P, Geo g, uint32_t dmask, uint32_t react,
+               int zchunk, int gx, int gy, int nblocks, int xcd)
+{
+  __shared__ BlockTables<T, NS> S;
+
+  int bid = blockIdx.x;
+  if (xcd) {
+    const int per = (nblocks + 7) / 8;
+    bid = (bid % 8) * per + bid / 8;
+  }
+  if (bid >= nblocks) return;
+  const int nthr = BX * blockDim.y;
+  const int tid = threadIdx.y * BX + threadIdx.x;
+  S.load(P, tid, nthr);
+  __syncthreads();
+
+  const int bx = bid % gx, by = (bid / gx) % gy, bz = bid / (gx * gy);
+  const int lane = threadIdx.x;
+  const int xw = (bx * (int)blockDim.y + (int)threadIdx.y) * BX;   /* first x of the wave */
+  if (xw >= g.nx) return;                                           /* whole wavefront     */
+  const int xr = xw + lane;
+  const bool xin = xr < g.nx;
+  const int x = xin ? xr : g.nx - 1;
+  /* lane 0 fetches x-1, lane 63 fetches x+1 (both clamped) */
+  const int xe = lane == 0 ? max(xr - 1, 0) : min(xr + 1, g.nx - 1);
+  const bool edge_lane = lane == 0 || lane == BX - 1;
+  const int y0 = by * Y;
+  const int z0 = bz * zchunk;
+  const int z1 = min(g.nz, z0 + zchunk);
+  if (z0 >= z1) return;
+
+  uint32_t roff[Y];   /* byte offset of (x, row r) within a species plane */
+#pragma unroll
+  for (int r = 0; r < Y; r++) roff[r] = ((uint32_t)min(y0 + r, g.ny - 1) * (uint32_t)g.nx + (uint32_t)x) * (uint32_t)sizeof(T);
+  const uint32_t hmoff = ((uint32_t)max(y0 - 1, 0) * (uint32_t)g.nx + (uint32_t)x) * (uint32_t)sizeof(T);
+  const uint32_t hpoff = ((uint32_t)min(y0 + Y, g.ny - 1) * (uint32_t)g.nx + (uint32_t)x) * (uint32_t)sizeof(T);
+  const int32_t edelta = (xe - x) * (int32_t)sizeof(T);
+
+  const int nb = react == 1 ? (int)P->nb : 0;
+  const uint32_t rbound = P->rbound;
+  const T dtr = P->dt_react;
+  const bool dirichlet = P->dirichlet != 0;
+  const bool xface = xr == 0 || xr + 1 == g.nx;
+  const int ztop_plane = g.nz - 1;
+  const size_t pstride = g.plane;   /* elements per plane */
+
+  /* descriptors over the planes this block touches (see k_stencil) */
+  const int zb = max(z0 - 1, 0), ze = min(g.nz, z1 + 2);
+  const uint32_t ES = (uint32_t)sizeof(T);
+  const uint32_t pbytes = (uint32_t)pstride * ES;
+  Rsrc rin[NS], rout[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    rin[s] = make_rsrc(u + s * g.nv + (size_t)zb * pstride, (uint32_t)(ze - zb) * pbytes);
+    rout[s] = make_rsrc(un + s * g.nv + (size_t)z0 * pstride, (uint32_t)(z1 - z0) * pbytes);
+  }
+  const Rsrc rid = make_rsrc(ids + (size_t)zb * pstride, (uint32_t)(ze - zb) * (uint32_t)pstride);
+  uint32_t vox[Y];    /* voxel index of (x, row r) within a plane */
+#pragma unroll
+  for (int r = 0; r < Y; r++) vox[r] = roff[r] / ES;
+
+  /* register ring over z (own rows): cm = z-1, cc = z, cp = z+1, cq = z+2 */
+  T cm[NS][Y], cc[NS][Y], cp[NS][Y], cq[NS][Y];
+#pragma unroll
+  for (int s = 0; s < NS; s++)
+#pragma unroll
+    for (int r = 0; r < Y; r++) {
+      cc[s][r] = bload<T>(rin[s], roff[r], (uint32_t)(z0 - zb) * pbytes);
+      cm[s][r] = bload<T>(rin[s], roff[r], (uint32_t)(max(z0 - 1, 0) - zb) * pbytes);
+      if (PF == 2) cp[s][r] = bload<T>(rin[s], roff[r], (uint32_t)(min(z0 + 1, ztop_plane) - zb) * pbytes);
+    }
+  /* Drain the prologue loads here.  Otherwise the wait the compiler places
+   * at the loop header for them also covers the previous plane's stores
+   * (stores count in vmcnt on gfx9), so every plane would wait for its own
+   * stores to complete. */
+  __builtin_amdgcn_s_waitcnt(0);
+  for (int z = z0; z < z1; z++) {
+    const uint32_t sz = (uint32_t)(z - zb) * pbytes;
+    const uint32_t szi = (uint32_t)(z - zb) * (uint32_t)pstride;
+    /* this plane's y halos and x edges first, then the planes ahead, so a
+     * wait for the former leaves the latter in flight */
+    T hm[NS], hp[NS], ex[NS][Y];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      hm[s] = cc[s][0];
+      hp[s] = cc[s][Y - 1];
+#pragma unroll
+      for (int r = 0; r < Y; r++) ex[s][r] = cc[s][r];
+      if ((dmask >> s) & 1u) {
+        hm[s] = bload<T>(rin[s], hmoff, sz);
+        hp[s] = bload<T>(rin[s], hpoff, sz);
+        if (edge_lane) {
+#pragma unroll
+          for (int r = 0; r < Y; r++) ex[s][r] = bload<T>(rin[s], (uint32_t)((int32_t)roff[r] + edelta), sz);
+        }
+      }
+    }
+    uint32_t idw[Y];
+#pragma unroll
+    for (int r = 0; r < Y; r++) idw[r] = bload_u8(rid, vox[r], szi);
+    if (PF == 1) {
+      const uint32_t sp = (uint32_t)(min(z + 1, ztop_plane) - zb) * pbytes;
+#pragma unroll
+      for (int s = 0; s < NS; s++)
+#pragma unroll
+        for (int r = 0; r < Y; r++) cp[s][r] = bload<T>(rin[s], roff[r], sp);
+    } else if (z + 1 < z1) {
+      const uint32_t sq = (uint32_t)(min(z + 2, ztop_plane) - zb) * pbytes;
+#pragma unroll
+      for (int s = 0; s < NS; s++)
+#pragma unroll
+        for (int r = 0; r < Y; r++) cq[s][r] = bload<T>(rin[s], roff[r], sq);
+    }
+
+    /* x neighbours of every row through DPP (all lanes active here) */
+    T xm[NS][Y], xp[NS][Y];
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+#pragma unroll
+      for (int r = 0; r < Y; r++) {
+        xm[s][r] = dpp_from_left(cc[s][r], ex[s][r]);
+        xp[s][r] = dpp_from_right(cc[s][r], ex[s][r]);
+      }
+    /* With one plane ahead, every value loaded this iteration is complete
+     * before the first store, so only stores are outstanding at the loop
+     * back edge and the ring copies need no wait there. */
+    if (PF == 1) {
+#pragma unroll
+      for (int s = 0; s < NS; s++)
+#pragma unroll
+        for (int r = 0; r < Y; r++) asm volatile("" ::"v"(cp[s][r]));
+    }
+
+#pragma unroll
+    for (int r = 0; r < Y; r++) {
+      const int yr = y0 + r;
+      const uint32_t vb = vox[r];
+      const int d = idw[r] & 7;
+      const bool uni = (idw[r] & 0x80) != 0;
+      int dxm = d, dxp = d, dym = d, dyp = d, dzm = d, dzp = d;
+      if (!uni) {
+        const int yc = min(yr, g.ny - 1);
+        dxm = bload_u8(rid, x > 0 ? vb - 1 : vb, szi) & 7;
+        dxp = bload_u8(rid, x + 1 < g.nx ? vb + 1 : vb, szi) & 7;
+        dym = bload_u8(rid, yc > 0 ? vb - (uint32_t)g.nx : vb, szi) & 7;
+        dyp = bload_u8(rid, yc + 1 < g.ny ? vb + (uint32_t)g.nx : vb, szi) & 7;
+        dzm = z > 0 ? bload_u8(rid, vb, szi - (uint32_t)pstride) & 7 : d;
+        dzp = z < ztop_plane ? bload_u8(rid, vb, szi + (uint32_t)pstride) & 7 : d;
+      }
+      T uc[NS], nw[NS];
+#pragma unroll
+      for (int s = 0; s < NS; s++) uc[s] = cc[s][r];
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        T acc = uc[s];
+        if ((dmask >> s) & 1u) {
+          const T c = uc[s];
+          const T ym = r > 0 ? cc[s][r - 1] : hm[s];
+          const T yp = r < Y - 1 ? cc[s][r + 1] : hp[s];
+          const T exm = xm[s][r] - c, exp_ = xp[s][r] - c;
+          const T eym = ym - c, eyp = yp - c;
+          const T ezm = cm[s][r] - c, ezp = cp[s][r] - c;
+          const T* C = S.coef[s] + d * MAXD;
+          if (uni) {
+            acc = c + C[d] * ((ezm + ezp) + (eym + eyp) + (exm + exp_));
+          } else {
+            acc = c + ((C[dzm] * ezm + C[dzp] * ezp) + (C[dym] * eym + C[dyp] * eyp) +
+                       (C[dxm] * exm + C[dxp] * exp_));
+          }
+        }
+        nw[s] = acc;
+      }
+      if (react == 2) react_species_major<T, NS>(nw, uc, S, d, rbound, dtr);
+      else react_general<T, NS>(nw, uc, S, d, nb, dmask, dtr);
+      if (dirichlet && (xface || yr == 0 || yr + 1 == g.ny || z == 0 || z == ztop_plane)) nw[0] = S.init[d][0];
+      const uint32_t fm = S.fixed[d];
+      if (fm) {
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+          if ((fm >> s) & 1u) nw[s] = S.init[d][s];
+      }
+      if (xin && yr < g.ny) {
+#pragma unroll
+        for (int s = 0; s < NS; s++) bstore<T>(rout[s], roff[r], (uint32_t)(z - z0) * pbytes, nw[s], NT);
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+#pragma unroll
+      for (int r = 0; r < Y; r++) {
+        cm[s][r] = cc[s][r];
+        cc[s][r] = cp[s][r];
+        if (PF == 2) cp[s][r] = cq[s][r];
+      }
   }
 }
 
@@ -628,6 +1021,50 @@ __global__ void k_stats_final(const double* __restrict__ partial, uint32_t nbloc
   out[3 * i + 0] = a; out[3 * i + 1] = b; out[3 * i + 2] = c;
 }
 
+/* Linescan reductions (species.c:1960-2035).  The reference's 3-D
+ * Gaussian weight is a product of one weight per axis, so the linescan of
+ * a species is a 1-D convolution of its weighted projection on the scan
+ * axis; these kernels form that projection's partial sums in FP64, over
+ * voxels whose domain bit is set in dmask, in a fixed order.
+ *
+ * k_linescan_rows: rows[z*ny + y] = sum_x u[z][y][x] wx[x]; one
+ * wavefront per (y, z) row, butterfly reduction across the wavefront. */
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_linescan_rows(const T* __restrict__ u, const uint8_t* __restrict__ ids, Geo g, uint32_t dmask,
+                const double* __restrict__ wx, double* __restrict__ rows)
+{
+  const uint64_t row = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  if (row >= (uint64_t)g.ny * (uint64_t)g.nz) return;   /* whole wavefront */
+  const int lane = threadIdx.x & 63;
+  const uint64_t base = row * (uint64_t)g.nx;
+  double acc = 0.0;
+  for (int x = lane; x < g.nx; x += 64)
+    if ((dmask >> (ids[base + x] & 7)) & 1u) acc += (double)u[base + x] * wx[x];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if (lane == 0) rows[row] = acc;
+}
+
+/* k_linescan_cols: part[(z*gridDim.y + c)*nx + x] = sum over the rows y
+ * of chunk c of u[z][y][x] wy[y]; one block per (plane, row chunk), a
+ * lane per x. */
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_linescan_cols(const T* __restrict__ u, const uint8_t* __restrict__ ids, Geo g, uint32_t dmask,
+                const double* __restrict__ wy, int ychunk, double* __restrict__ part)
+{
+  const int z = blockIdx.x, c = blockIdx.y;
+  const int y0 = c * ychunk, y1 = min(g.ny, y0 + ychunk);
+  for (int x = threadIdx.x; x < g.nx; x += blockDim.x) {
+    double acc = 0.0;
+    for (int y = y0; y < y1; y++) {
+      const uint64_t v = ((uint64_t)z * g.ny + y) * g.nx + x;
+      if ((dmask >> (ids[v] & 7)) & 1u) acc += (double)u[v] * wy[y];
+    }
+    part[((uint64_t)z * gridDim.y + c) * g.nx + x] = acc;
+  }
+}
+
 /* Plain copy of a known number of bytes (16 B per lane per iteration),
  * used to calibrate memory-traffic counters against algorithmic bytes. */
 __global__ void k_calibration_copy(const uint4* __restrict__ a, uint4* __restrict__ b, uint64_t n16)
@@ -682,13 +1119,16 @@ struct cs_context {
   /* reductions */
   double* d_partial = nullptr;
   double* d_stats = nullptr;
+  double* d_lweights = nullptr;   /* linescan weights, nx + ny + nz        */
+  double* d_lpart = nullptr;      /* linescan partial sums                  */
   /* host copies */
   cs_problem prob{};
   std::vector<uint32_t> state_off, npairs;
   std::vector<uint8_t> init_states;
   uint32_t substeps[MAXS] = {0};
   uint32_t flux_substep[MAXF] = {0};
-  LaunchCfg launch{4, 32, 1, 1, 0, 0, 0};
+  LaunchCfg launch{4, 32, 1, 1, 0, 0, 0, 0};
+  uint32_t rfast = 0;   /* buffer reactions admit the species-major form */
 };
 
 namespace {
@@ -706,7 +1146,7 @@ constexpr size_t LDS_TILE_LIMIT = 60 * 1024;
  * times over, z chunks no shorter than a few planes. */
 LaunchCfg default_launch(const Geo& g)
 {
-  LaunchCfg L{4, 32, 1, 1, 0, 0, 0};
+  LaunchCfg L{4, 32, 1, 1, 0, 0, 0, 0};
   const uint64_t xy = (uint64_t)((g.nx + BX - 1) / BX) * ((g.ny + L.rows - 1) / L.rows);
   int chunks = (int)std::max<uint64_t>(1, (4096 + xy - 1) / xy);
   L.zchunk = std::max(4, (g.nz + chunks - 1) / chunks);
@@ -720,7 +1160,8 @@ void free_ctx(cs_context* c)
   hipSetDevice(c->device);
   void* ptrs[] = {c->d_u[0], c->d_u[1], c->d_ids, c->d_params, c->d_pairs, c->d_states,
                   c->d_row_voxel, c->d_row_species, c->d_row_start, c->d_ent_pair,
-                  c->d_ent_flux, c->d_ent_sign, c->d_uniforms, c->d_partial, c->d_stats};
+                  c->d_ent_flux, c->d_ent_sign, c->d_uniforms, c->d_partial, c->d_stats,
+                  c->d_lweights, c->d_lpart};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -752,6 +1193,28 @@ void fill_params(DevParams<T>& dp, const cs_problem* p, const cs_context* c)
       dp.init[d][s] = (T)p->init[d][s];
       if (p->fixed[d][s]) dp.fixed_mask[d] |= 1u << s;
     }
+  /* species-major table: usable when every buffer slot in use binds
+   * species 0 (its free species) and no domain binds a species twice */
+  bool fast = p->num_buffers > 0;
+  for (int d = 0; d < MAXD; d++) {
+    uint32_t seen = 0;
+    for (int b = 0; b < MAXS; b++) {
+      const bool used = p->bsp0[b][d] || p->bsp1[b][d] || p->k_on[b][d] != 0 || p->k_off[b][d] != 0 ||
+                        p->tot[b][d] != 0;
+      if (!used) continue;
+      const int b0 = p->bsp0[b][d];
+      if (b >= (int)p->num_buffers || p->bsp1[b][d] != 0 || b0 == 0 || ((seen >> b0) & 1u)) {
+        fast = false;
+        continue;
+      }
+      seen |= 1u << b0;
+      dp.rt[d][b0].kon = (T)p->k_on[b][d];
+      dp.rt[d][b0].koff = (T)p->k_off[b][d];
+      dp.rt[d][b0].tot = (T)p->tot[b][d];
+      dp.rbound |= 1u << b0;
+    }
+  }
+  dp.rfast = fast ? 1u : 0u;
   dp.dt_react = (T)p->dt_react;
   dp.nb = p->num_buffers;
   dp.dirichlet = p->force_dirichlet;
@@ -797,6 +1260,10 @@ int validate(const cs_problem* p)
     return set_err(CS_EINVAL, "num_domains %u outside 1..%d", p->num_domains, MAXD);
   if (p->precision != CS_PREC_F64 && p->precision != CS_PREC_F32)
     return set_err(CS_EINVAL, "unknown precision %u", p->precision);
+  /* the stencil kernels address a plane with 32-bit byte offsets */
+  const uint64_t plane_bytes = (uint64_t)p->n[0] * p->n[1] * (p->precision == CS_PREC_F64 ? 8 : 4);
+  if (plane_bytes >= (1ull << 31)) return set_err(CS_ERANGE, "an x-y plane of %llu bytes exceeds 2^31",
+                                                 (unsigned long long)plane_bytes);
   if (!p->domain_ids) return set_err(CS_EINVAL, "domain_ids is NULL");
   if (p->num_buffers > MAXS) return set_err(CS_EINVAL, "too many buffers");
   if (p->num_fluxes > MAXF) return set_err(CS_EINVAL, "too many fluxes");
@@ -835,6 +1302,7 @@ int upload_params(cs_context* c)
 {
   DevParams<T> dp;
   fill_params<T>(dp, &c->prob, c);
+  c->rfast = dp.rfast;
   if (!c->d_params) HIP_TRY(hipMalloc(&c->d_params, sizeof(DevParams<T>)));
   HIP_TRY(hipMemcpyAsync(c->d_params, &dp, sizeof dp, hipMemcpyHostToDevice, c->stream));
   return CS_OK;
@@ -863,6 +1331,21 @@ void launch_stencil_v(cs_context* c, const LaunchCfg& L, const void* uin, void* 
   const bool lin = L.lin && !L.lds;
   const int gz = (g.nz + L.zchunk - 1) / L.zchunk;
   int gx, gy;
+  if (L.yb) {
+    gx = (g.nx + BX * L.rows - 1) / (BX * L.rows);
+    gy = (g.ny + L.yb - 1) / L.yb;
+    const int nblocks = gx * gy * gz;
+    const int launch = L.xcd ? ((nblocks + 7) / 8) * 8 : nblocks;
+    if (L.yb == 4)
+      hipLaunchKernelGGL((k_stencil_rows<T, NS, 4, PF, NT>), dim3(launch), dim3(BX, L.rows), 0, c->stream,
+                         (const T*)uin, (T*)uout, c->d_ids, (const DevParams<T>*)c->d_params, g, dmask,
+                         react, L.zchunk, gx, gy, nblocks, L.xcd);
+    else
+      hipLaunchKernelGGL((k_stencil_rows<T, NS, 2, PF, NT>), dim3(launch), dim3(BX, L.rows), 0, c->stream,
+                         (const T*)uin, (T*)uout, c->d_ids, (const DevParams<T>*)c->d_params, g, dmask,
+                         react, L.zchunk, gx, gy, nblocks, L.xcd);
+    return;
+  }
   if (lin) {
     gx = (int)((g.plane + (uint64_t)BX * L.rows - 1) / ((uint64_t)BX * L.rows));
     gy = 1;
@@ -888,7 +1371,7 @@ template <typename T, int NS>
 void launch_stencil_ns(cs_context* c, const LaunchCfg& L, const void* uin, void* uout, uint32_t dmask,
                        uint32_t react)
 {
-  if (L.pf == 2 && !L.lds) {
+  if (L.pf == 2 && (!L.lds || L.yb)) {
     if (L.nt) launch_stencil_v<T, NS, 2, true>(c, L, uin, uout, dmask, react);
     else launch_stencil_v<T, NS, 2, false>(c, L, uin, uout, dmask, react);
   } else {
@@ -919,6 +1402,14 @@ void launch_stencil(cs_context* c, uint32_t dmask, uint32_t react)
   launch_stencil_cfg<T>(c, c->launch, c->d_u[c->cur], c->d_u[c->cur ^ 1], dmask, react);
 }
 
+/* Buffer reactions of step ti: 0 none, 1 in the reference's slot order,
+ * 2 in species-major form (allowed when species 0 diffuses this step). */
+uint32_t react_mode(const cs_context* c, uint64_t ti, uint32_t dmask)
+{
+  if (c->prob.num_buffers == 0 || ti % c->prob.reaction_substep != 0) return 0;
+  return (c->rfast && (dmask & 1u)) ? 2u : 1u;
+}
+
 /* One step: gate (if due), stencil+reaction, fluxes; then swap. */
 template <typename T>
 int step_one(cs_context* c, uint64_t ti, const double* d_uni, uint64_t uni_base, hipEvent_t e0,
@@ -928,7 +1419,7 @@ int step_one(cs_context* c, uint64_t ti, const double* d_uni, uint64_t uni_base,
   uint32_t dmask = 0;
   for (uint32_t s = 0; s < c->ns; s++)
     if (c->substeps[s] && ti % c->substeps[s] == 0) dmask |= 1u << s;
-  const uint32_t react = (p.num_buffers > 0 && ti % p.reaction_substep == 0) ? 1u : 0u;
+  const uint32_t react = react_mode(c, ti, dmask);
   uint32_t fluxmask = 0;
   for (uint32_t f = 0; f < c->nf; f++)
     if (ti % c->flux_substep[f] == 0) fluxmask |= 1u << f;
@@ -1316,6 +1807,76 @@ int cs_reduce_stats(cs_context* c, double* mn, double* mx, double* tot)
   return CS_OK;
 }
 
+int cs_linescan(cs_context* c, uint32_t species, uint32_t axis, const double* wx, const double* wy,
+                const double* wz, uint32_t domain_mask, double* proj)
+{
+  if (!c || !proj) return set_err(CS_EINVAL, "NULL argument");
+  if (species >= c->ns) return set_err(CS_ERANGE, "species %u out of range", species);
+  if (axis > 2) return set_err(CS_EINVAL, "axis must be 0, 1 or 2");
+  if ((axis != 0 && !wx) || (axis != 1 && !wy) || (axis != 2 && !wz))
+    return set_err(CS_EINVAL, "the weights of both transverse axes are required");
+  HIP_TRY(hipSetDevice(c->device));
+  const Geo& g = c->geo;
+  const int ychunk = 64;
+  const int gyc = (g.ny + ychunk - 1) / ychunk;
+  const size_t nrows = (size_t)g.ny * g.nz, ncols = (size_t)g.nz * gyc * g.nx;
+  if (!c->d_lweights) HIP_TRY(hipMalloc(&c->d_lweights, sizeof(double) * ((size_t)g.nx + g.ny + g.nz)));
+  if (!c->d_lpart) HIP_TRY(hipMalloc(&c->d_lpart, sizeof(double) * std::max(nrows, ncols)));
+  double* dwx = c->d_lweights;
+  double* dwy = dwx + g.nx;
+  const void* us = (const char*)c->d_u[c->cur] + (size_t)species * g.nv * c->esize;
+  std::vector<double> part;
+  if (axis == 0) {
+    HIP_TRY(hipMemcpyAsync(dwy, wy, sizeof(double) * g.ny, hipMemcpyHostToDevice, c->stream));
+    if (c->esize == 8)
+      hipLaunchKernelGGL(k_linescan_cols<double>, dim3(g.nz, gyc), dim3(256), 0, c->stream, (const double*)us,
+                         c->d_ids, g, domain_mask, dwy, ychunk, c->d_lpart);
+    else
+      hipLaunchKernelGGL(k_linescan_cols<float>, dim3(g.nz, gyc), dim3(256), 0, c->stream, (const float*)us,
+                         c->d_ids, g, domain_mask, dwy, ychunk, c->d_lpart);
+    HIP_TRY(hipGetLastError());
+    part.resize(ncols);
+    HIP_TRY(hipMemcpyAsync(part.data(), c->d_lpart, sizeof(double) * ncols, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int x = 0; x < g.nx; x++) {
+      double acc = 0.0;
+      for (int z = 0; z < g.nz; z++) {
+        double col = 0.0;
+        for (int k = 0; k < gyc; k++) col += part[((size_t)z * gyc + k) * g.nx + x];
+        acc += wz[z] * col;
+      }
+      proj[x] = acc;
+    }
+    return CS_OK;
+  }
+  HIP_TRY(hipMemcpyAsync(dwx, wx, sizeof(double) * g.nx, hipMemcpyHostToDevice, c->stream));
+  const unsigned blocks = (unsigned)((nrows + 3) / 4);
+  if (c->esize == 8)
+    hipLaunchKernelGGL(k_linescan_rows<double>, dim3(blocks), dim3(256), 0, c->stream, (const double*)us, c->d_ids,
+                       g, domain_mask, dwx, c->d_lpart);
+  else
+    hipLaunchKernelGGL(k_linescan_rows<float>, dim3(blocks), dim3(256), 0, c->stream, (const float*)us, c->d_ids,
+                       g, domain_mask, dwx, c->d_lpart);
+  HIP_TRY(hipGetLastError());
+  part.resize(nrows);
+  HIP_TRY(hipMemcpyAsync(part.data(), c->d_lpart, sizeof(double) * nrows, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (axis == 1) {
+    for (int y = 0; y < g.ny; y++) {
+      double acc = 0.0;
+      for (int z = 0; z < g.nz; z++) acc += wz[z] * part[(size_t)z * g.ny + y];
+      proj[y] = acc;
+    }
+  } else {
+    for (int z = 0; z < g.nz; z++) {
+      double acc = 0.0;
+      for (int y = 0; y < g.ny; y++) acc += wy[y] * part[(size_t)z * g.ny + y];
+      proj[z] = acc;
+    }
+  }
+  return CS_OK;
+}
+
 int cs_get_channel_states(cs_context* c, uint32_t f, uint8_t* states)
 {
   if (!c || !states) return set_err(CS_EINVAL, "NULL argument");
@@ -1350,14 +1911,23 @@ int cs_get_launch_config(const cs_context* c, cs_launch_config* out)
   out->nontemporal = (uint32_t)c->launch.nt;
   out->lds_tile = (uint32_t)c->launch.lds;
   out->linear = (uint32_t)c->launch.lin;
+  out->yblock = (uint32_t)c->launch.yb;
   return CS_OK;
 }
 
 int cs_set_launch_config(cs_context* c, const cs_launch_config* in)
 {
   if (!c || !in) return set_err(CS_EINVAL, "NULL argument");
-  if (!(in->rows == 1 || in->rows == 2 || in->rows == 4 || in->rows == 8))
+  if (in->yblock) {
+    if (in->yblock != 2 && in->yblock != 4) return set_err(CS_EINVAL, "yblock must be 0, 2 or 4");
+    if (in->rows < 1 || in->rows > 8) return set_err(CS_EINVAL, "the row-blocked kernel takes 1 to 8 wavefronts per block");
+    if (in->lds_tile || in->linear) return set_err(CS_EINVAL, "yblock excludes lds_tile and linear");
+    if (c->geo.plane * c->esize >= (1ull << 31))
+      return set_err(CS_EINVAL, "plane of %llu voxels too large for the row-blocked kernel",
+                     (unsigned long long)c->geo.plane);
+  } else if (!(in->rows == 1 || in->rows == 2 || in->rows == 4 || in->rows == 8)) {
     return set_err(CS_EINVAL, "rows must be 1, 2, 4 or 8");
+  }
   if (in->zchunk < 1) return set_err(CS_EINVAL, "zchunk must be >= 1");
   if (in->prefetch != 1 && in->prefetch != 2) return set_err(CS_EINVAL, "prefetch must be 1 or 2");
   if (in->xcd_remap > 1 || in->nontemporal > 1 || in->lds_tile > 1 || in->linear > 1)
@@ -1374,6 +1944,7 @@ int cs_set_launch_config(cs_context* c, const cs_launch_config* in)
   L.nt = (int)in->nontemporal;
   L.lds = (int)in->lds_tile;
   L.lin = (int)in->linear;
+  L.yb = (int)in->yblock;
   c->launch = L;
   return CS_OK;
 }
@@ -1387,60 +1958,80 @@ int cs_autotune(cs_context* c, uint64_t first_time_ind, uint32_t reps, cs_launch
   HIP_TRY(hipSetDevice(c->device));
   /* the substep pattern of the coming steps: which species diffuse and
    * whether the reactions fire differs from step to step, and so does the
-   * best shape, so every candidate is timed over the same run of steps */
+   * best shape.  Each distinct (diffusing species, reactions) pair of one
+   * period is timed once per candidate and weighted by how often it occurs */
   uint64_t period = 1;
   auto lcm = [](uint64_t a, uint64_t b) { uint64_t x = a, y = b; while (y) { uint64_t t = x % y; x = y; y = t; } return a / x * b; };
   for (uint32_t s = 0; s < c->ns; s++)
-    if (c->substeps[s]) period = std::min<uint64_t>(lcm(period, c->substeps[s]), 240);
-  if (c->prob.num_buffers) period = std::min<uint64_t>(lcm(period, c->prob.reaction_substep), 240);
-  std::vector<uint32_t> masks, reacts;
-  for (uint64_t k = 0; k < period * reps; k++) {
+    if (c->substeps[s]) period = std::min<uint64_t>(lcm(period, c->substeps[s]), 100000);
+  if (c->prob.num_buffers) period = std::min<uint64_t>(lcm(period, c->prob.reaction_substep), 100000);
+  struct Pattern { uint32_t mask, react; uint64_t count; };
+  std::vector<Pattern> pats;
+  for (uint64_t k = 0; k < period; k++) {
     const uint64_t ti = first_time_ind + k;
     uint32_t dm = 0;
     for (uint32_t s = 0; s < c->ns; s++)
       if (c->substeps[s] && ti % c->substeps[s] == 0) dm |= 1u << s;
-    masks.push_back(dm);
-    reacts.push_back((c->prob.num_buffers > 0 && ti % c->prob.reaction_substep == 0) ? 1u : 0u);
+    const uint32_t re = react_mode(c, ti, dm);
+    bool found = false;
+    for (auto& q : pats)
+      if (q.mask == dm && q.react == re) { q.count++; found = true; break; }
+    if (!found) pats.push_back({dm, re, 1});
   }
+  const int nz = c->geo.nz;
+  std::vector<int> zcs;
+  for (int zc : {8, 16, 25, 32, 50, 64})
+    if (zc < nz) zcs.push_back(zc);
+  zcs.push_back(nz);
+  std::vector<LaunchCfg> cands;
+  for (int zc : zcs)
+    for (int xcd : {0, 1})
+      for (int nt : {0, 1}) {
+        for (int pf : {1, 2})
+          for (int rows : {2, 4, 8}) {
+            cands.push_back({rows, zc, xcd, pf, nt, 0, 0, 0});   /* 2-D tiles      */
+            cands.push_back({rows, zc, xcd, pf, nt, 0, 1, 0});   /* linear columns */
+          }
+        for (int rows : {2, 4, 8})
+          if (lds_tile_bytes(c->esize, c->ns, rows) <= LDS_TILE_LIMIT)
+            cands.push_back({rows, zc, xcd, 1, nt, 1, 0, 0});    /* LDS tiles      */
+        if (c->geo.plane * c->esize < (1ull << 31)) {
+          const int full = std::min(8, (c->geo.nx + BX - 1) / BX);
+          std::vector<int> wv{full};
+          if (full > 2) wv.push_back((full + 1) / 2);
+          for (int yb : {2, 4})
+            for (int pf : {1, 2})
+              for (int w : wv) cands.push_back({w, zc, xcd, pf, nt, 0, 0, yb});   /* row-blocked */
+        }
+      }
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
   LaunchCfg winner = c->launch;
-  float wms = 1e30f;
-  const int nz = c->geo.nz;
-  std::vector<int> zcs;
-  for (int zc : {8, 16, 32, 64})
-    if (zc < nz) zcs.push_back(zc);
-  zcs.push_back(nz);
+  double wms = 1e30;
   int rc = CS_OK;
-  for (int kind : {0, 1, 2})   /* 2-D tiles, linear columns, LDS tiles */
-  for (int rows : {2, 4, 8})
-    for (int zc : zcs)
-      for (int xcd : {0, 1})
-        for (int pf : {1, 2})
-          for (int nt : {0, 1}) {
-            const int lds = kind == 2, lin = kind == 1;
-            if (lds && pf != 1) continue;
-            if (lds && lds_tile_bytes(c->esize, c->ns, rows) > LDS_TILE_LIMIT) continue;
-            LaunchCfg L{rows, zc, xcd, pf, nt, lds, lin};
-            /* reads the current state, writes the other buffer (scratch
-             * until the next step overwrites it); the state is unchanged */
-            auto go = [&](uint32_t dm, uint32_t re) {
-              if (c->esize == 8) launch_stencil_cfg<double>(c, L, c->d_u[c->cur], c->d_u[c->cur ^ 1], dm, re);
-              else launch_stencil_cfg<float>(c, L, c->d_u[c->cur], c->d_u[c->cur ^ 1], dm, re);
-            };
-            go(masks[0], reacts[0]);
-            if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = set_err(CS_EDEVICE, "event record failed"); goto done; }
-            for (size_t k = 0; k < masks.size(); k++) go(masks[k], reacts[k]);
-            if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
-              rc = set_err(CS_EDEVICE, "autotune launch failed: %s", hipGetErrorString(hipGetLastError()));
-              goto done;
-            }
-            float ms = 0;
-            hipEventElapsedTime(&ms, e0, e1);
-            ms /= (float)masks.size();
-            if (ms < wms) { wms = ms; winner = L; }
-          }
+  for (const LaunchCfg& L : cands) {
+    /* reads the current state, writes the other buffer (scratch until the
+     * next step overwrites it); the state is unchanged */
+    auto go = [&](uint32_t dm, uint32_t re) {
+      if (c->esize == 8) launch_stencil_cfg<double>(c, L, c->d_u[c->cur], c->d_u[c->cur ^ 1], dm, re);
+      else launch_stencil_cfg<float>(c, L, c->d_u[c->cur], c->d_u[c->cur ^ 1], dm, re);
+    };
+    double mean = 0;
+    for (const Pattern& q : pats) {
+      go(q.mask, q.react);
+      if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = set_err(CS_EDEVICE, "event record failed"); goto done; }
+      for (uint32_t r = 0; r < reps; r++) go(q.mask, q.react);
+      if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+        rc = set_err(CS_EDEVICE, "autotune launch failed: %s", hipGetErrorString(hipGetLastError()));
+        goto done;
+      }
+      float ms = 0;
+      hipEventElapsedTime(&ms, e0, e1);
+      mean += (double)ms / reps * (double)q.count / (double)period;
+    }
+    if (mean < wms) { wms = mean; winner = L; }
+  }
 done:
   hipEventDestroy(e0);
   hipEventDestroy(e1);

@@ -527,10 +527,9 @@ static int write_output(run_ctx* R, uint32_t output_ind, double t, uint64_t time
   }
   /* linescan (species.c:1960-2102): the 3-D Gaussian factorises, so the
    * field is first projected on the scan axis with the two transverse
-   * Gaussians, then convolved along the axis */
+   * Gaussians (on the device), then convolved along the axis here */
   if (a->linescan) {
     linescan_arg* l = a->linescan;
-    cs_try(cs_download_species(R->ctx, (uint32_t)R->linescan_species, R->field), "cs_download_species");
     const double h = S->problem.h;
     const int dir = l->axis;
     double off[3];
@@ -541,20 +540,12 @@ static int write_output(run_ctx* R, uint32_t output_ind, double t, uint64_t time
       for (uint64_t k = 0; k < N[ax]; k++) g[ax][k] = gauss(off[ax] - (double)k * h, R->model->conv[ax]);
     }
     double* proj = (double*)calloc(N[dir], sizeof(double));
-    const uint8_t* ids = S->problem.domain_ids;
-    for (uint64_t x = 0; x < N[0]; x++)
-      for (uint64_t y = 0; y < N[1]; y++)
-        for (uint64_t z = 0; z < N[2]; z++) {
-          const uint64_t v = (x * N[1] + y) * N[2] + z;
-          /* the reference keeps voxels whose domain index is below the
-           * number of listed domains (species.c:2015-2024) */
-          if (l->num_domains && ids[v] >= (uint8_t)l->num_domains) continue;
-          const uint64_t c[3] = {x, y, z};
-          double w = 1.0;
-          for (int ax = 0; ax < 3; ax++)
-            if (ax != dir) w *= g[ax][c[ax]];
-          proj[c[dir]] += R->field[v] * w;
-        }
+    /* the reference keeps the voxels whose domain id is below the number
+     * of listed domains (species.c:2015-2024) */
+    const uint32_t mask = !l->num_domains ? 0xFFFFFFFFu
+                          : l->num_domains >= 8 ? 0xFFu : ((1u << l->num_domains) - 1u);
+    cs_try(cs_linescan(R->ctx, (uint32_t)R->linescan_species, (uint32_t)dir, g[0], g[1], g[2], mask, proj),
+           "cs_linescan");
     double* line = (double*)calloc(N[dir], sizeof(double));
     for (uint64_t p = 0; p < N[dir]; p++) {
       double sum = 0;

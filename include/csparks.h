@@ -146,6 +146,19 @@ int cs_synchronize(cs_context* ctx);
 int cs_reduce_stats(cs_context* ctx, double* min_values, double* max_values,
                     double* total_values);
 
+/* Weighted projection of one species on a reference axis (0 x, 1 y, 2 z)
+ * for the linescan of Species_compute_convolution_with_gaussian_function_
+ * locally (species.c:1960-2035): proj[k] = sum over the voxels whose
+ * coordinate on `axis` is k and whose domain id d has bit d set in
+ * domain_mask, of u * w_a * w_b, with w_a, w_b the weights of the two other
+ * axes at the voxel (wx, wy, wz have n[0], n[1], n[2] entries; the one of
+ * `axis` is not read and may be NULL).  The reference's 3-D Gaussian is
+ * the product of one Gaussian per axis, so its linescan is the 1-D
+ * convolution of this projection along the axis.  FP64 sums in a fixed
+ * order; the field stays on the device. */
+int cs_linescan(cs_context* ctx, uint32_t species, uint32_t axis, const double* wx,
+                const double* wy, const double* wz, uint32_t domain_mask, double* proj);
+
 /* Channel states of discrete flux f (0 closed, 1 open). */
 int cs_get_channel_states(cs_context* ctx, uint32_t flux, uint8_t* states);
 int cs_set_channel_states(cs_context* ctx, uint32_t flux, const uint8_t* states);
@@ -163,19 +176,26 @@ int cs_time_steps(cs_context* ctx, uint64_t first_time_ind, uint64_t nsteps,
  * state past the caches; lds_tile stages each plane in LDS so the x/y
  * neighbours are read from LDS instead of memory; linear hands each
  * wavefront 64 consecutive (x,y) columns of the plane across row ends, so
- * no lane idles when nx is not a multiple of 64.  No reference counterpart: the reference's
- * OpenMP/AVX-512 loop has no launch shape (kernel_generic.cpp:561-600). */
+ * no lane idles when nx is not a multiple of 64; yblock > 0 selects the
+ * row-blocked kernel, in which every lane owns yblock consecutive rows (2
+ * or 4), a block is `rows` wavefronts side by side along x (1 to 8) and x
+ * neighbours move between lanes by DPP.  No reference counterpart: the
+ * reference's OpenMP/AVX-512 loop has no launch shape
+ * (kernel_generic.cpp:561-600). */
 typedef struct cs_launch_config {
-  uint32_t rows, zchunk, xcd_remap, prefetch, nontemporal, lds_tile, linear;
+  uint32_t rows, zchunk, xcd_remap, prefetch, nontemporal, lds_tile, linear, yblock;
 } cs_launch_config;
 
 int cs_get_launch_config(const cs_context* ctx, cs_launch_config* cfg);
 int cs_set_launch_config(cs_context* ctx, const cs_launch_config* cfg);
 
-/* Time every candidate launch shape on the context's grid over reps
- * periods of the substep pattern starting at first_time_ind (reading the
- * current state and writing the spare buffer, so the state is unchanged)
- * and keep the fastest; best_ms is its mean kernel time. */
+/* Time every candidate launch shape on the context's grid and keep the
+ * fastest.  The steps from first_time_ind over one period of the substep
+ * pattern are grouped by (diffusing species, reactions); each group is
+ * launched reps times per candidate and weighted by its share of the
+ * period.  Launches read the current state and write the spare buffer, so
+ * the state is unchanged; best_ms is the winner's weighted mean kernel
+ * time. */
 int cs_autotune(cs_context* ctx, uint64_t first_time_ind, uint32_t reps,
                 cs_launch_config* best, double* best_ms);
 

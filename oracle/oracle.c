@@ -555,6 +555,45 @@ void oracle_set_species(oracle_sim* s, uint32_t sp, const double* in)
   memcpy(s->du[sp], in, sizeof(double) * s->nv);
 }
 
+/* gauss, utils.c:209-213; the constants are 1/sqrt(2) and 2/sqrt(pi)
+ * (math.h's M_SQRT1_2 and M_2_SQRTPI, hidden under -std=c11) */
+static double oracle_gauss(double x, double sigma)
+{
+  const double sqrt1_2 = 0.70710678118654752440, two_sqrtpi = 1.12837916709551257390;
+  const double inv_sqrt_sigma = 1.0 / sqrt(sigma);
+  return sqrt1_2 * two_sqrtpi * inv_sqrt_sigma / 2 * exp(-x * x / 2 / sigma);
+}
+
+/* Linescan, species.c:1960-2035 with the offsets of species.c:856-877:
+ * for every index p along `axis`, the sum over all voxels of
+ * u * gauss(o_x - x) * gauss(o_y - y) * gauss(o_z - z), where a coordinate
+ * is index * h, o = (N/2 + offset) * h on the two other axes and p * h on
+ * `axis`.  With num_domains > 0 a voxel counts only if its domain id is
+ * below num_domains: the reference compares the id with the loop counter
+ * over the listed domains, not with the listed ids (species.c:2015-2024).
+ * Evaluated directly, O(N[axis] * voxels). */
+void oracle_linescan(const oracle_sim* s, uint32_t sp, uint32_t axis, const int32_t offsets[3],
+                     uint32_t num_domains, const double conv[3], double* out)
+{
+  double off[3];
+  for (int ax = 0; ax < 3; ax++) off[ax] = (double)((int64_t)(s->n[ax] / 2) + offsets[ax]) * s->h;
+  const double* u = s->u1[sp];
+  for (uint64_t p = 0; p < s->n[axis]; p++) {
+    off[axis] = (double)p * s->h;
+    double sum = 0;
+    for (uint64_t x = 0; x < s->n[0]; x++)
+      for (uint64_t y = 0; y < s->n[1]; y++)
+        for (uint64_t z = 0; z < s->n[2]; z++) {
+          const uint64_t v = (x * s->n[1] + y) * s->n[2] + z;
+          if (num_domains && s->ids[v] >= num_domains) continue;
+          sum += u[v] * oracle_gauss(off[0] - (double)x * s->h, conv[0]) *
+                 oracle_gauss(off[1] - (double)y * s->h, conv[1]) *
+                 oracle_gauss(off[2] - (double)z * s->h, conv[2]);
+        }
+    out[p] = sum;
+  }
+}
+
 void oracle_stats(const oracle_sim* s, double* mn, double* mx, double* tot)
 {
   uint32_t nds = s->nd * s->ns;

@@ -62,6 +62,10 @@ void oracle_set_species(oracle_sim* s, uint32_t species, const double* in);
  * (Species_output_scalar_data, species.c:2806-2863). */
 void oracle_stats(const oracle_sim* s, double* mn, double* mx, double* tot);
 
+/* Linescan of one species (species.c:1960-2035): out[N[axis]]. */
+void oracle_linescan(const oracle_sim* s, uint32_t species, uint32_t axis, const int32_t offsets[3],
+                     uint32_t num_domains, const double conv[3], double* out);
+
 /* Channel states of discrete boundary k (file order among discrete). */
 int  oracle_get_states(const oracle_sim* s, uint32_t k, uint8_t* out);
 int  oracle_set_states(oracle_sim* s, uint32_t k, const uint8_t* in);

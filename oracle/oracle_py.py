@@ -58,6 +58,9 @@ def lib() -> C.CDLL:
         L.oracle_set_species.restype = None
         L.oracle_stats.argtypes = [P] + [C.POINTER(C.c_double)] * 3
         L.oracle_stats.restype = None
+        L.oracle_linescan.argtypes = [P, C.c_uint32, C.c_uint32, C.POINTER(C.c_int32), C.c_uint32,
+                                      C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.oracle_linescan.restype = None
         L.oracle_get_states.argtypes = [P, C.c_uint32, C.POINTER(C.c_uint8)]
         L.oracle_set_states.argtypes = [P, C.c_uint32, C.POINTER(C.c_uint8)]
         L.oracle_ryr_flux.argtypes = [C.c_double] * 4 + [C.POINTER(C.c_double)]
@@ -125,6 +128,14 @@ class Oracle:
         mn, mx, tot = np.empty(k), np.empty(k), np.empty(k)
         lib().oracle_stats(self.ptr, dptr(mn), dptr(mx), dptr(tot))
         return mn.reshape(self.nd, self.ns), mx.reshape(self.nd, self.ns), tot.reshape(self.nd, self.ns)
+
+    def linescan(self, s: int, axis: int, offsets, conv, num_domains: int = 0) -> np.ndarray:
+        """Linescan of species s along axis (species.c:1960-2035)."""
+        off = (C.c_int32 * 3)(*[int(o) for o in offsets])
+        cv = np.ascontiguousarray(conv, dtype=np.float64)
+        out = np.empty(self.n[axis], dtype=np.float64)
+        lib().oracle_linescan(self.ptr, s, axis, off, num_domains, dptr(cv), dptr(out))
+        return out
 
     def states(self, k: int) -> np.ndarray:
         n = int(self.info.discrete_size[k])

@@ -64,7 +64,7 @@ def test_cli_end_to_end_matches_oracle(tmp_path):
     gp, mp = _inputs(tmp_path, d)
     r = _run_cli(["-g", gp, "-m", mp, "-t", "0.01", "-o", "0.002", "-S", "3", "-C", "ryr", "0.005",
                   "-c", "run", "-s", "Ca", "Fluo", "-z", "30", "--all_data", "Ca", "CMDN-CSQN",
-                  "-O", "ryr", str(d.open_index)], tmp_path)
+                  "-O", "ryr", str(d.open_index), "-l", "x", "Fluo", "1", "-2"], tmp_path)
     out = str(tmp_path / "run.h5")
     opts = cs.Options(seed=3, t_close={"ryr": 0.005}, open={"ryr": [d.open_index]})
     orc = Oracle(d.geometry, d.model, opts)
@@ -94,6 +94,10 @@ def test_cli_end_to_end_matches_oracle(tmp_path):
     assert ok, err
     sheet = cs.read_output_dataset(out, f"/data_{last_out:07d}/Fluo_z_sheet_5")
     np.testing.assert_allclose(sheet, orc.species(1)[:, :, 5], rtol=1e-10)
+    # -l x Fluo 1 -2: offsets 1 on y and -2 on z (options.c:154-208)
+    line = cs.read_output_dataset(out, f"/data_{last_out:07d}/linescan_Fluo_y_1_z_-2")
+    want = orc.linescan(1, 0, (0, 1, -2), d.model.conv)
+    np.testing.assert_allclose(line, want, rtol=1e-10, atol=1e-10 * float(np.max(np.abs(want))))
     assert f"{last_out}: time:" in r.stdout and "conservation" in r.stdout
 
 

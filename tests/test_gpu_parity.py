@@ -93,6 +93,41 @@ def test_fp64_serca_and_dirichlet():
     sim.close()
 
 
+def _with_model(d, **changes):
+    import dataclasses
+    return dyad.Dyad(d.geometry, dataclasses.replace(d.model, **changes), d.open_index, d.description)
+
+
+def test_reactions_in_slot_order_when_a_buffer_binds_another_species():
+    """A cleft buffer whose free species is Fluo (not species 0) rules out
+    the species-major form: every step takes the reference's slot-order
+    path (reaction_SIMD, kernel_generic.cpp:397-509)."""
+    d = dyad.small_dyad(seed=14)
+    bufs = [list(b) for b in d.model.buffers]
+    bufs[1][3] = cs.Buffer(4, 1, 0.0327, 0.0196, 70.0)
+    d = _with_model(d, buffers=bufs)
+    opts = cs.Options(seed=2, open={"ryr": [d.open_index]}, t_close={"ryr": 1.0})
+    sim, orc = _run_pair(d, opts, 120, chunks=2)
+    _compare_fields(sim, orc, RTOL_F64)
+    sim.close()
+
+
+def test_reactions_overwrite_when_species0_does_not_diffuse():
+    """Fluo diffuses fastest, so species 0 diffuses on a substep only; on
+    the other steps every buffer slot overwrites species 0 with its own
+    increment instead of accumulating (reaction_SIMD with update_species[0]
+    unset, kernel_generic.cpp:413-427); the GPU must do the same."""
+    d = dyad.small_dyad(seed=15)
+    m = d.model
+    sigma = [[{0: 22000.0, 1: 220000.0}.get(s, sg) for s, sg in zip(dl, sl)]
+             for dl, sl in zip(m.diffusive, m.sigma)]
+    d = _with_model(d, sigma=sigma)
+    opts = cs.Options(seed=2, open={"ryr": [d.open_index]}, t_close={"ryr": 1.0})
+    sim, orc = _run_pair(d, opts, 120, chunks=2)
+    _compare_fields(sim, orc, RTOL_F64)
+    sim.close()
+
+
 @pytest.mark.parametrize("rng", ["philox", "mt"])
 def test_stochastic_trajectories_match(rng):
     """Same uniforms on both sides => identical channel trajectories."""
@@ -151,6 +186,11 @@ LAUNCHES = [dict(rows=8, zchunk=3, xcd_remap=1, prefetch=2, nontemporal=1, lds_t
             dict(rows=4, zchunk=3, xcd_remap=1, prefetch=2, nontemporal=1, lds_tile=0, linear=1),
             dict(rows=1, zchunk=14, xcd_remap=0, prefetch=1, nontemporal=0, lds_tile=0, linear=1),
             dict(rows=8, zchunk=6, xcd_remap=0, prefetch=1, nontemporal=1, lds_tile=0, linear=1)]
+LAUNCHES = [{**c, "yblock": 0} for c in LAUNCHES] + [
+    dict(rows=1, zchunk=3, xcd_remap=1, prefetch=1, nontemporal=1, lds_tile=0, linear=0, yblock=2),
+    dict(rows=2, zchunk=64, xcd_remap=0, prefetch=2, nontemporal=0, lds_tile=0, linear=0, yblock=4),
+    dict(rows=1, zchunk=1, xcd_remap=0, prefetch=2, nontemporal=1, lds_tile=0, linear=0, yblock=4),
+]
 
 
 @pytest.mark.parametrize("cfg", LAUNCHES)
@@ -167,6 +207,62 @@ def test_launch_shapes_agree(cfg):
     orc.step(60)
     _compare_fields(sim, orc, RTOL_F64)
     sim.close()
+
+
+ROW_LAUNCHES = [dict(rows=1, zchunk=2, xcd_remap=0, prefetch=1, nontemporal=0, yblock=2),
+                dict(rows=2, zchunk=4, xcd_remap=1, prefetch=1, nontemporal=1, yblock=4),
+                dict(rows=3, zchunk=9, xcd_remap=0, prefetch=2, nontemporal=1, yblock=2),
+                dict(rows=8, zchunk=5, xcd_remap=1, prefetch=2, nontemporal=0, yblock=4)]
+
+
+@pytest.mark.parametrize("cfg", ROW_LAUNCHES)
+def test_row_kernel_wide_grid(cfg):
+    """The row-blocked kernel on a grid 150 wide (three wavefronts, the
+    last one partial, x neighbours crossing wavefront and block edges) with
+    13 rows (not a multiple of 2 or 4) and 9 planes."""
+    d = with_serca(dyad.small_dyad(n=(150, 13, 9), seed=21, num_ryr=12, num_pryr=4))
+    opts = cs.Options(seed=6, open={"ryr": [d.open_index]}, t_close={"ryr": 1.0})
+    _say(f"case: row kernel {cfg} on {d.description}")
+    orc = Oracle(d.geometry, d.model, opts)
+    sim = cs.Simulation(d.geometry, d.model, opts)
+    sim.set_launch_config(lds_tile=0, linear=0, **cfg)
+    sim.step(40)
+    orc.step(40)
+    _compare_fields(sim, orc, RTOL_F64)
+    sim.close()
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_linescan_matches_oracle(precision):
+    """Device projection + host convolution (cs_linescan) against the
+    oracle's direct sum of species.c:1960-2035, every axis, with and
+    without the domain rule."""
+    d = with_serca(dyad.small_dyad(seed=16))
+    opts = cs.Options(seed=2, open={"ryr": [d.open_index]}, t_close={"ryr": 1.0}, precision=precision)
+    sim, orc = _run_pair(d, opts, 60)
+    tol = RTOL_F64 if precision == "f64" else RTOL_F32
+    for s, axis, offsets, nd in [(1, 0, (0, 2, -3), 0), (0, 1, (1, 0, 2), 2), (1, 2, (-2, 3, 0), 1),
+                                 (2, 2, (0, 0, 0), 0), (0, 0, (0, -4, 1), 3)]:
+        _say(f"linescan species {s} axis {axis} offsets {offsets} domains {nd}")
+        got = sim.linescan(s, axis, offsets, d.model.conv, nd)
+        want = orc.linescan(s, axis, offsets, d.model.conv, nd)
+        np.testing.assert_allclose(got, want, rtol=tol, atol=tol * float(np.max(np.abs(want))))
+    sim.close()
+    orc.close()
+
+
+def test_linescan_wide_grid():
+    """x scans over more lanes than a block has and y in several chunks."""
+    d = dyad.small_dyad(n=(300, 70, 6), seed=22, num_ryr=16, num_pryr=4)
+    opts = cs.Options(seed=3, open={"ryr": [d.open_index]})
+    sim, orc = _run_pair(d, opts, 10)
+    for axis, offsets in [(0, (0, 5, -1)), (1, (-20, 0, 1)), (2, (7, -3, 0))]:
+        _say(f"wide linescan axis {axis}")
+        got = sim.linescan(0, axis, offsets, d.model.conv)
+        want = orc.linescan(0, axis, offsets, d.model.conv)
+        np.testing.assert_allclose(got, want, rtol=RTOL_F64, atol=RTOL_F64 * float(np.max(np.abs(want))))
+    sim.close()
+    orc.close()
 
 
 def test_autotune_keeps_state():

@@ -480,3 +480,95 @@ def test_refined_density_pairs_expand_to_face():
     got = {(np.unravel_index(F.pairs[2 * k], s.n), np.unravel_index(F.pairs[2 * k + 1], s.n)) for k in range(9)}
     want = {((3 * x0 + a, 3 * y0 + b, 3 * z0 + 2), (3 * x1 + a, 3 * y1 + b, 3 * z1)) for a in range(3) for b in range(3)}
     assert got == want
+
+
+# ------------------------------------------------------------- linescan
+# species.c:1960-2035 with gauss (utils.c:209-213) =
+# exp(-x^2 / (2 sigma)) / sqrt(2 pi sigma), the normal density of variance
+# sigma; scipy's norm.pdf is the independent check.
+
+def _normal(x, var):
+    from scipy.stats import norm
+    return norm.pdf(x, scale=math.sqrt(var))
+
+
+def _linescan_oracle(seed=5):
+    d = dyad.small_dyad(seed=seed)
+    orc = op.Oracle(d.geometry, d.model, cs.Options(seed=1))
+    return d, orc
+
+
+CONV = (2900.0, 3700.0, 9100.0)
+
+
+@pytest.mark.parametrize("axis,offsets", [(0, (0, 2, -3)), (1, (1, 0, 2)), (2, (-2, 3, 0))])
+def test_linescan_of_a_point_is_the_product_of_normal_densities(axis, offsets):
+    d, orc = _linescan_oracle()
+    n, h = orc.n, orc.info.h
+    f = np.zeros(n)
+    i, j, k = 7, 5, 9
+    f[i, j, k] = 2.5
+    orc.set_species(1, f)
+    got = orc.linescan(1, axis, offsets, CONV)
+    o = [(n[ax] // 2 + offsets[ax]) * h for ax in range(3)]
+    c = (i * h, j * h, k * h)
+    want = []
+    for p in range(n[axis]):
+        oo = list(o)
+        oo[axis] = p * h
+        want.append(2.5 * np.prod([_normal(oo[ax] - c[ax], CONV[ax]) for ax in range(3)]))
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=0)
+    orc.close()
+
+
+def test_linescan_of_a_constant_field_factorises():
+    d, orc = _linescan_oracle()
+    n, h = orc.n, orc.info.h
+    orc.set_species(2, np.full(n, 3.0))
+    offsets = (1, -1, 2)
+    got = orc.linescan(2, 1, offsets, CONV)
+    sx = _normal((n[0] // 2 + offsets[0]) * h - np.arange(n[0]) * h, CONV[0]).sum()
+    sz = _normal((n[2] // 2 + offsets[2]) * h - np.arange(n[2]) * h, CONV[2]).sum()
+    want = [3.0 * sx * sz * _normal(p * h - np.arange(n[1]) * h, CONV[1]).sum() for p in range(n[1])]
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=0)
+    orc.close()
+
+
+def test_linescan_domain_rule_compares_ids_with_the_count():
+    """With k listed domains the reference keeps the voxels whose id is
+    below k (species.c:2015-2024), whatever the listed names are."""
+    d, orc = _linescan_oracle()
+    ids = np.asarray(d.geometry.voxels)
+    assert ids.shape == tuple(orc.n)
+    rng = np.random.default_rng(3)
+    f = rng.random(orc.n)
+    orc.set_species(0, f)
+    full = orc.linescan(0, 2, (0, 0, 0), CONV)
+    for k in (1, 2, 3):
+        orc.set_species(0, np.where(ids < k, f, 0.0))
+        want = orc.linescan(0, 2, (0, 0, 0), CONV)
+        orc.set_species(0, f)
+        np.testing.assert_allclose(orc.linescan(0, 2, (0, 0, 0), CONV, num_domains=k), want, rtol=1e-13)
+    orc.set_species(0, np.where(ids >= 1, f, 0.0))
+    assert np.all(orc.linescan(0, 0, (0, 0, 0), CONV, num_domains=1) == 0.0)
+    assert np.any(full > 0)
+    orc.close()
+
+
+def test_linescan_separable_evaluation_matches_direct_sum():
+    """Projection with the transverse weights, then a 1-D convolution --
+    the order the device path uses -- equals the direct triple sum."""
+    d, orc = _linescan_oracle(seed=6)
+    n, h = orc.n, orc.info.h
+    f = np.random.default_rng(4).random(n)
+    orc.set_species(3, f)
+    for axis, offsets in [(0, (0, 1, 1)), (1, (2, 0, -1)), (2, (0, 0, 0))]:
+        o = [(n[ax] // 2 + offsets[ax]) * h for ax in range(3)]
+        w = [_normal(o[ax] - np.arange(n[ax]) * h, CONV[ax]) for ax in range(3)]
+        others = [ax for ax in range(3) if ax != axis]
+        proj = np.moveaxis(f, axis, 0)
+        proj = np.tensordot(proj, np.multiply.outer(w[others[0]], w[others[1]]), axes=([1, 2], [0, 1]))
+        k = np.arange(n[axis]) * h
+        want = [float(np.dot(proj, _normal(p - k, CONV[axis]))) for p in k]
+        np.testing.assert_allclose(orc.linescan(3, axis, offsets, CONV), want, rtol=1e-12)
+    orc.close()
