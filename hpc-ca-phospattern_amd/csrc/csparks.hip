@@ -339,10 +339,9 @@ k_stencil(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict
   /* clamped neighbour offsets: a missing neighbour reads the voxel
    * itself, so its (u_n - u) term vanishes -- the no-flux outer faces.
    * A lane's column is fixed while it marches z, so its offsets within a
-   * plane are computed once, as 32-bit byte offsets from a per-plane,
-   * per-species base address that is uniform across the wavefront (the
-   * host checks plane * sizeof(T) < 2^31); the loop does no per-lane
-   * address arithmetic. */
+   * plane are computed once (32-bit byte offsets; the host checks
+   * plane * sizeof(T) < 2^31); the loop does no per-lane address
+   * arithmetic. */
   const uint32_t ES = (uint32_t)sizeof(T);
   const uint32_t col = (uint32_t)y * (uint32_t)g.nx + (uint32_t)x;
   const uint32_t bc = col * ES;
@@ -364,19 +363,12 @@ k_stencil(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict
   }
   const Rsrc rid = make_rsrc(ids + (size_t)zb * plane, (uint32_t)(ze - zb) * (uint32_t)plane);
 
-  /* register ring over z: um = z-1, uc = z, up = z+1, upp = z+2 (PF 2);
-   * plane indices are clamped, so a missing z neighbour equals uc */
-  T um[NS], uc[NS], up[NS], upp[NS];
-#pragma unroll
-  for (int s = 0; s < NS; s++) {
-    uc[s] = bload<T>(rin[s], bc, (uint32_t)(z0 - zb) * pbytes);
-    um[s] = bload<T>(rin[s], bc, (uint32_t)(max(z0 - 1, 0) - zb) * pbytes);
-    if (PF == 2) up[s] = bload<T>(rin[s], bc, (uint32_t)(min(z0 + 1, ztop_plane) - zb) * pbytes);
-  }
-  /* drain the prologue loads, so the loop header's wait does not also
-   * cover the previous plane's stores (stores count in vmcnt on gfx9) */
-  __builtin_amdgcn_s_waitcnt(0);
-  for (int z = z0; z < z1; z++) {
+  /* One plane: um, uc, up hold planes z-1, z, z+1 (indices clamped, so a
+   * missing z neighbour equals uc).  With PF 1 up is loaded here; with
+   * PF 2 it was loaded one plane earlier and uq (plane z+2) is loaded here,
+   * after this plane's neighbours, so the wait for the neighbours (mostly
+   * cache hits: plane z arrived a plane ago) leaves uq in flight. */
+  auto plane_step = [&](const int z, const T (&um)[NS], const T (&uc)[NS], T (&up)[NS], T (&uq)[NS]) {
     const uint32_t sz = (uint32_t)(z - zb) * pbytes;           /* plane z in rin */
     const uint32_t szi = (uint32_t)(z - zb) * (uint32_t)plane; /* plane z in rid */
     const bool ztop = z < ztop_plane;
@@ -384,61 +376,71 @@ k_stencil(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict
       const uint32_t sp = (uint32_t)(min(z + 1, ztop_plane) - zb) * pbytes;
 #pragma unroll
       for (int s = 0; s < NS; s++) up[s] = bload<T>(rin[s], bc, sp);
-    } else if (z + 2 <= z1) {
-      const uint32_t sq = (uint32_t)(min(z + 2, ztop_plane) - zb) * pbytes;
-#pragma unroll
-      for (int s = 0; s < NS; s++) upp[s] = bload<T>(rin[s], bc, sq);
     }
     /* the in-plane neighbours of every diffusing species are loaded before
      * any of them is used, so one wait covers them all */
     const uint32_t id = bload_u8(rid, col, szi);
     T nxm[NS], nxp[NS], nym[NS], nyp[NS];
 #pragma unroll
-    for (int s = 0; s < NS; s++) {
-      nxm[s] = nxp[s] = nym[s] = nyp[s] = uc[s];
+    for (int s = 0; s < NS; s++)
       if ((dmask >> s) & 1u) {
         nxm[s] = bload<T>(rin[s], bxm, sz);
         nxp[s] = bload<T>(rin[s], bxp, sz);
         nym[s] = bload<T>(rin[s], bym, sz);
         nyp[s] = bload<T>(rin[s], byp, sz);
       }
-    }
-    /* with one plane ahead, the next plane is complete before the first
-     * store, so only stores are outstanding at the loop back edge and the
-     * ring copies below need no wait there */
-    if (PF == 1) {
+    if (PF == 2) {
+      asm volatile("" ::: "memory");   /* keep the plane-ahead loads last */
+      const uint32_t sq = (uint32_t)(min(z + 2, ztop_plane) - zb) * pbytes;
+#pragma unroll
+      for (int s = 0; s < NS; s++) uq[s] = bload<T>(rin[s], bc, sq);
+    } else {
+      /* the next plane is complete before the first store, so only stores
+       * are outstanding between planes */
 #pragma unroll
       for (int s = 0; s < NS; s++) asm volatile("" ::"v"(up[s]));
     }
     const int d = id & 7;
     const bool uni = (id & 0x80) != 0;
-    int dxm = d, dxp = d, dym = d, dyp = d, dzm = d, dzp = d;
-    if (!uni) {
-      dxm = bload_u8(rid, bxm / ES, szi) & 7;
-      dxp = bload_u8(rid, bxp / ES, szi) & 7;
-      dym = bload_u8(rid, bym / ES, szi) & 7;
-      dyp = bload_u8(rid, byp / ES, szi) & 7;
-      dzm = z > 0 ? bload_u8(rid, col, szi - (uint32_t)plane) & 7 : d;
-      dzp = ztop ? bload_u8(rid, col, szi + (uint32_t)plane) & 7 : d;
-    }
     T nw[NS];
 #pragma unroll
-    for (int s = 0; s < NS; s++) {
-      T acc = uc[s];
-      if ((dmask >> s) & 1u) {
-        const T c = uc[s];
-        const T exm = nxm[s] - c, exp_ = nxp[s] - c;
-        const T eym = nym[s] - c, eyp = nyp[s] - c;
-        const T ezm = um[s] - c, ezp = up[s] - c;
-        const T* C = S.coef[s] + d * MAXD;
-        if (uni) {
-          acc = c + C[d] * ((ezm + ezp) + (eym + eyp) + (exm + exp_));
-        } else {
-          acc = c + ((C[dzm] * ezm + C[dzp] * ezp) + (C[dym] * eym + C[dyp] * eyp) +
-                     (C[dxm] * exm + C[dxp] * exp_));
+    for (int s = 0; s < NS; s++) nw[s] = uc[s];
+    if (__ballot(!uni) == 0) {
+      /* every lane's six neighbours share its domain (the common case,
+       * decided once per wavefront) */
+#pragma unroll
+      for (int s = 0; s < NS; s++)
+        if ((dmask >> s) & 1u) {
+          const T c = uc[s];
+          const T exm = nxm[s] - c, exp_ = nxp[s] - c;
+          const T eym = nym[s] - c, eyp = nyp[s] - c;
+          const T ezm = um[s] - c, ezp = up[s] - c;
+          nw[s] = c + S.coef[s][d * MAXD + d] * ((ezm + ezp) + (eym + eyp) + (exm + exp_));
         }
+    } else {
+      int dxm = d, dxp = d, dym = d, dyp = d, dzm = d, dzp = d;
+      if (!uni) {
+        dxm = bload_u8(rid, bxm / ES, szi) & 7;
+        dxp = bload_u8(rid, bxp / ES, szi) & 7;
+        dym = bload_u8(rid, bym / ES, szi) & 7;
+        dyp = bload_u8(rid, byp / ES, szi) & 7;
+        dzm = z > 0 ? bload_u8(rid, col, szi - (uint32_t)plane) & 7 : d;
+        dzp = ztop ? bload_u8(rid, col, szi + (uint32_t)plane) & 7 : d;
       }
-      nw[s] = acc;
+#pragma unroll
+      for (int s = 0; s < NS; s++)
+        if ((dmask >> s) & 1u) {
+          const T c = uc[s];
+          const T exm = nxm[s] - c, exp_ = nxp[s] - c;
+          const T eym = nym[s] - c, eyp = nyp[s] - c;
+          const T ezm = um[s] - c, ezp = up[s] - c;
+          const T* C = S.coef[s] + d * MAXD;
+          if (uni)
+            nw[s] = c + C[d] * ((ezm + ezp) + (eym + eyp) + (exm + exp_));
+          else
+            nw[s] = c + ((C[dzm] * ezm + C[dzp] * ezp) + (C[dym] * eym + C[dyp] * eyp) +
+                         (C[dxm] * exm + C[dxp] * exp_));
+        }
     }
     if (react == 2) react_species_major<T, NS>(nw, uc, S, d, rbound, dtr);
     else react_general<T, NS>(nw, uc, S, d, nb, dmask, dtr);
@@ -451,11 +453,34 @@ k_stencil(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict
     }
     const uint32_t so = (uint32_t)(z - z0) * pbytes;
 #pragma unroll
-    for (int s = 0; s < NS; s++) {
-      bstore<T>(rout[s], bc, so, nw[s], NT);
-      um[s] = uc[s];
-      uc[s] = up[s];
-      if (PF == 2) up[s] = upp[s];
+    for (int s = 0; s < NS; s++) bstore<T>(rout[s], bc, so, nw[s], NT);
+  };
+
+  /* register ring of PF + 2 planes, rotated by renaming: the loop body
+   * holds every position of the ring, so no values are copied; planes past
+   * z1 in the last pass are skipped by uniform branches */
+  T ra[NS], rb[NS], rc[NS], rd[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    ra[s] = bload<T>(rin[s], bc, (uint32_t)(max(z0 - 1, 0) - zb) * pbytes);
+    rb[s] = bload<T>(rin[s], bc, (uint32_t)(z0 - zb) * pbytes);
+    if (PF == 2) rc[s] = bload<T>(rin[s], bc, (uint32_t)(min(z0 + 1, ztop_plane) - zb) * pbytes);
+  }
+  /* drain the prologue loads, so the loop header's wait does not also
+   * cover the previous plane's stores (stores count in vmcnt on gfx9) */
+  __builtin_amdgcn_s_waitcnt(0);
+  if (PF == 1) {
+    for (int z = z0; z < z1; z += 3) {
+      plane_step(z, ra, rb, rc, rd);
+      if (z + 1 < z1) plane_step(z + 1, rb, rc, ra, rd);
+      if (z + 2 < z1) plane_step(z + 2, rc, ra, rb, rd);
+    }
+  } else {
+    for (int z = z0; z < z1; z += 4) {
+      plane_step(z, ra, rb, rc, rd);
+      if (z + 1 < z1) plane_step(z + 1, rb, rc, rd, ra);
+      if (z + 2 < z1) plane_step(z + 2, rc, rd, ra, rb);
+      if (z + 3 < z1) plane_step(z + 3, rd, ra, rb, rc);
     }
   }
 }

@@ -172,7 +172,8 @@ int cs_time_steps(cs_context* ctx, uint64_t first_time_ind, uint64_t nsteps,
 /* Launch shape of the fused stencil/reaction kernel: blocks of 64 x rows
  * threads march zchunk planes; xcd_remap orders tiles so each of the 8
  * XCDs works on a contiguous slab (shared L2 for halo rows); prefetch is
- * the number of planes loaded ahead (1 or 2); nontemporal streams the next
+ * the number of planes loaded ahead (1 or 2; 1 with lds_tile);
+ * nontemporal streams the next
  * state past the caches; lds_tile stages each plane in LDS so the x/y
  * neighbours are read from LDS instead of memory; linear hands each
  * wavefront 64 consecutive (x,y) columns of the plane across row ends, so

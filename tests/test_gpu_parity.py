@@ -178,13 +178,13 @@ def test_fp32_against_fp64_oracle():
 LAUNCHES = [dict(rows=8, zchunk=3, xcd_remap=1, prefetch=2, nontemporal=1, lds_tile=0, linear=0),
             dict(rows=1, zchunk=1, xcd_remap=0, prefetch=1, nontemporal=0, lds_tile=0, linear=0),
             dict(rows=2, zchunk=5, xcd_remap=1, prefetch=1, nontemporal=1, lds_tile=0, linear=0),
-            dict(rows=4, zchunk=64, xcd_remap=0, prefetch=2, nontemporal=0, lds_tile=0, linear=0),
+            dict(rows=4, zchunk=64, xcd_remap=0, prefetch=1, nontemporal=0, lds_tile=0, linear=0),
             dict(rows=4, zchunk=4, xcd_remap=1, prefetch=1, nontemporal=1, lds_tile=1, linear=0),
             dict(rows=8, zchunk=64, xcd_remap=0, prefetch=1, nontemporal=0, lds_tile=1, linear=0),
             dict(rows=1, zchunk=2, xcd_remap=0, prefetch=1, nontemporal=0, lds_tile=1, linear=0),
             dict(rows=2, zchunk=7, xcd_remap=1, prefetch=1, nontemporal=1, lds_tile=1, linear=0),
             dict(rows=4, zchunk=3, xcd_remap=1, prefetch=2, nontemporal=1, lds_tile=0, linear=1),
-            dict(rows=1, zchunk=14, xcd_remap=0, prefetch=1, nontemporal=0, lds_tile=0, linear=1),
+            dict(rows=1, zchunk=14, xcd_remap=0, prefetch=2, nontemporal=0, lds_tile=0, linear=1),
             dict(rows=8, zchunk=6, xcd_remap=0, prefetch=1, nontemporal=1, lds_tile=0, linear=1)]
 LAUNCHES = [{**c, "yblock": 0} for c in LAUNCHES] + [
     dict(rows=1, zchunk=3, xcd_remap=1, prefetch=1, nontemporal=1, lds_tile=0, linear=0, yblock=2),
