@@ -37,6 +37,8 @@ BYTES_F64 = 1 + 5 * (8 + 8)
 METRIC = "spark realisations/sec (closure vs plain) at KS<0.05; stencil HBM GB/s %peak"
 TSTOP_MS = 30.0
 PROFILE_TRAFFIC = os.path.join(REPO, "profiles", "traffic.json")
+WORKLOAD_KEY = "cluster_dyad"
+LAUNCH_KEYS = ["rows", "zchunk", "xcd_remap", "prefetch", "nontemporal", "lds_tile", "linear", "yblock"]
 
 
 def log(msg):
@@ -83,12 +85,12 @@ def main():
                           "GBps": 2 * (args.copy_calibration << 20) / (ms * 1e-3) / 1e9}))
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "
-              "(independent realisations would shard by instance, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported: the domain stays "
+              "resident on one GPU (independent realisations would shard by instance)", file=sys.stderr)
         sys.exit(2)
 
     t0 = time.perf_counter()
-    d = dyad.config1(seed=1, n=tuple(args.size))
+    d = dyad.cluster_dyad(seed=1, n=tuple(args.size))
     opts = cs.Options(h=3.0, seed=1, open={"ryr": [0]}, precision=args.precision, tstop=TSTOP_MS)
     sim = cs.Simulation(d.geometry, d.model, opts)
     log(f"setup {time.perf_counter() - t0:.1f}s; {cs.device_info()}; grid {sim.n}")
@@ -99,8 +101,7 @@ def main():
     sim.synchronize()
     launch = sim.launch_config()
     if args.launch:
-        keys = ["rows", "zchunk", "xcd_remap", "prefetch", "nontemporal", "lds_tile", "linear", "yblock"]
-        sim.set_launch_config(**dict(zip(keys, (int(v) for v in args.launch.split(",")))))
+        sim.set_launch_config(**dict(zip(LAUNCH_KEYS, (int(v) for v in args.launch.split(",")))))
         launch = sim.launch_config()
     elif not args.no_autotune:
         t2 = time.perf_counter()
@@ -121,12 +122,17 @@ def main():
     tot = sim.stats()[2].sum()
     sim.close()
 
+    # HBM bytes per launch of this kernel and launch shape, from the
+    # committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by the copy
+    # kernel's readings); null when no profile matches
     traffic = None
     if os.path.exists(PROFILE_TRAFFIC):
         with open(PROFILE_TRAFFIC) as f:
             t = json.load(f)
-        if t.get("workload") == "config1" and t.get("precision") == args.precision and tuple(t.get("grid", ())) == tuple(args.size):
-            traffic = t.get("bytes_per_launch")
+        key = ",".join(str(launch[k]) for k in LAUNCH_KEYS)
+        if t.get("workload") == WORKLOAD_KEY and t.get("precision") == args.precision and \
+                tuple(t.get("grid", ())) == tuple(args.size):
+            traffic = t.get("bytes_per_launch", {}).get(key)
 
     cpu = None
     if not args.no_cpu:
@@ -148,8 +154,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f32" if args.precision == "f32" else "f64",
         "data": "synthetic",
-        "config": {"workload": "config1: single 400x400x200 dyad @3nm, 7 species/5 slots, 100 RyR "
-                               "(30% pRyR), stochastic gating, FP32 fields",
+        "config": {"workload": f"single {args.size[0]}x{args.size[1]}x{args.size[2]} dyad @3nm, 7 species in "
+                               "5 slots, 100 RyR (30% pRyR), stochastic Philox gating, "
+                               f"{'FP32' if args.precision == 'f32' else 'FP64'} fields",
                    "grid": list(args.size), "voxels": nv,
                    "bytes_per_voxel_step": bpv},
         "frac_of_peak": round(gbs / PEAK_HBM_GBS, 4),

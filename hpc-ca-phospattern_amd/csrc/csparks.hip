@@ -1167,14 +1167,12 @@ size_t lds_tile_bytes(size_t esize, uint32_t ns, int rows)
 }
 constexpr size_t LDS_TILE_LIMIT = 60 * 1024;
 
-/* Default launch shape: enough blocks to keep every CU busy several
- * times over, z chunks no shorter than a few planes. */
+/* Default launch shape, the fastest measured on MI355X for the 32M-voxel
+ * single dyad in FP32: linear columns, blocks of 8 wavefronts, 12 planes
+ * per block, one plane ahead, non-temporal stores of the next state. */
 LaunchCfg default_launch(const Geo& g)
 {
-  LaunchCfg L{4, 32, 1, 1, 0, 0, 0, 0};
-  const uint64_t xy = (uint64_t)((g.nx + BX - 1) / BX) * ((g.ny + L.rows - 1) / L.rows);
-  int chunks = (int)std::max<uint64_t>(1, (4096 + xy - 1) / xy);
-  L.zchunk = std::max(4, (g.nz + chunks - 1) / chunks);
+  LaunchCfg L{8, 12, 0, 1, 1, 0, 1, 0};
   L.zchunk = std::min(L.zchunk, g.nz);
   return L;
 }

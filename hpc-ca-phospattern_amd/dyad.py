@@ -152,7 +152,7 @@ class Dyad:
     description: str
 
 
-def config0(seed: int = 1, num_pryr: int = 10) -> Dyad:
+def lattice_dyad(seed: int = 1, num_pryr: int = 10) -> Dyad:
     """384x384x150 nm at 6 nm (64x64x25): T-tubule 0-30 nm, cleft 30-42 nm
     and jSR 42-72 nm under the central 240x240 nm; 49 RyRs on a 7x7 lattice
     at 30 nm pitch, num_pryr of them phosphorylated (chosen by seed); the
@@ -167,10 +167,10 @@ def config0(seed: int = 1, num_pryr: int = 10) -> Dyad:
     geom = dyad_geometry(lay, pts, mask)
     ryr_idx = [i for i in range(len(pts)) if not mask[i]]
     model = dyad_species_model(with_pryr=num_pryr > 0)
-    return Dyad(geom, model, ryr_idx.index(centre), "config0 64x64x25 @6nm, 49 RyR")
+    return Dyad(geom, model, ryr_idx.index(centre), "lattice dyad 64x64x25 @6nm, 49 RyR")
 
 
-def config1(seed: int = 1, n: Tuple[int, int, int] = (400, 400, 200), h: float = 3.0,
+def cluster_dyad(seed: int = 1, n: Tuple[int, int, int] = (400, 400, 200), h: float = 3.0,
             num_ryr: int = 100, pryr_fraction: float = 0.3) -> Dyad:
     """1200x1200x600 nm at 3 nm (400x400x200 = 32M voxels): cleft 12 nm
     (4 voxels), jSR slab 30 nm over a 600x600 nm footprint, 100 RyRs by
@@ -193,13 +193,13 @@ def config1(seed: int = 1, n: Tuple[int, int, int] = (400, 400, 200), h: float =
     mask = [i in chosen for i in range(num_ryr)]
     geom = dyad_geometry(lay, pts, mask)
     model = dyad_species_model()
-    return Dyad(geom, model, 0, f"config1 {nx}x{ny}x{nz} @{h}nm, {num_ryr} RyR")
+    return Dyad(geom, model, 0, f"cluster dyad {nx}x{ny}x{nz} @{h}nm, {num_ryr} RyR")
 
 
 def small_dyad(n: Tuple[int, int, int] = (20, 18, 14), h: float = 6.0, seed: int = 3,
                num_ryr: int = 9, num_pryr: int = 3) -> Dyad:
     """A few-thousand-voxel dyad for parity tests that the oracle finishes
-    in seconds; same structure as config0."""
+    in seconds; same structure as lattice_dyad."""
     nx, ny, nz = n
     mx, my = max(1, min(3, nx // 4)), max(1, min(3, ny // 4))
     lay = DyadLayout(n=n, h=h, tt=2, cleft=2, jsr=3, footprint=(mx, nx - mx, my, ny - my))

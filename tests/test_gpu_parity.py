@@ -306,20 +306,20 @@ def test_upload_download_roundtrip():
     sim.close()
 
 
-def test_config1_size_conservation():
+def test_cluster_dyad_conservation():
     """Full 400x400x200 grid (32M voxels): with no-flux faces and internal
     fluxes only, total calcium (free + bound, every species slot) is
     conserved; FP64 must keep it to round-off."""
-    d = dyad.config1(seed=1)
+    d = dyad.cluster_dyad(seed=1)
     opts = cs.Options(h=3.0, seed=1, open={"ryr": list(range(0, 60, 3))}, t_close={"ryr": 5.0})
-    _say("case: config1 conservation fp64")
+    _say("case: 400x400x200 cluster dyad conservation fp64")
     sim = cs.Simulation(d.geometry, d.model, opts)
     assert sim.n == (400, 400, 200)
     tot0 = sim.stats()[2].sum()
     sim.step(50)
     mn, mx, tot = sim.stats()
     rel = abs(tot.sum() - tot0) / tot0
-    _say(f"config1 conservation rel drift {rel:.2e}")
+    _say(f"cluster dyad conservation rel drift {rel:.2e}")
     assert rel < 1e-12
     assert np.all(mn[:3] > -1e-9)
     sim.close()

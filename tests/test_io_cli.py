@@ -27,7 +27,7 @@ def test_g1_fixture_matches_reference_counts():
 
 
 def test_geometry_species_roundtrip(tmp_path):
-    d = with_serca(dyad.config0(seed=3))
+    d = with_serca(dyad.lattice_dyad(seed=3))
     gp, mp = str(tmp_path / "g.h5"), str(tmp_path / "m.h5")
     cs.write_geometry(gp, d.geometry)
     cs.write_species(mp, d.model)
@@ -95,7 +95,7 @@ def test_cli_requires_geometry(tmp_path):
 
 
 def test_cli_rejects_open_index_beyond_boundary(tmp_path):
-    d = dyad.config0()
+    d = dyad.lattice_dyad()
     gp, mp = _write_inputs(tmp_path, d)
     r = _cli(["-g", gp, "-m", mp, "-q", "-O", "ryr", "3", "60"], tmp_path)
     assert r.returncode == 1
@@ -103,14 +103,14 @@ def test_cli_rejects_open_index_beyond_boundary(tmp_path):
 
 
 def test_cli_rejects_non_increasing_open_indices(tmp_path):
-    d = dyad.config0()
+    d = dyad.lattice_dyad()
     gp, mp = _write_inputs(tmp_path, d)
     r = _cli(["-g", gp, "-m", mp, "-O", "ryr", "5", "2"], tmp_path)
     assert r.returncode == 1 and "monotonicly" in r.stderr
 
 
 def test_cli_rejects_unknown_save_species(tmp_path):
-    d = dyad.config0()
+    d = dyad.lattice_dyad()
     gp, mp = _write_inputs(tmp_path, d)
     r = _cli(["-g", gp, "-m", mp, "-q", "-s", "Nope"], tmp_path)
     assert r.returncode == 1 and '"Nope" cannot be found in' in r.stderr

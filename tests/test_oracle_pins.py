@@ -417,7 +417,7 @@ def test_gating_matches_reference_functions(reff):
 
 # ------------------------------------------------------------- builder
 def test_builder_matches_oracle_setup():
-    for d, h in [(dyad.config0(), 6.0), (dyad.small_dyad(n=(10, 9, 7), h=12.0), 6.0),
+    for d, h in [(dyad.lattice_dyad(), 6.0), (dyad.small_dyad(n=(10, 9, 7), h=12.0), 6.0),
                  (dyad.small_dyad(n=(8, 8, 6), h=12.0), 3.0)]:
         opts = cs.Options(h=h, seed=5, dt_update_react=1e-4, dt_update_stoch=6e-5)
         s = cs.Setup(d.geometry, d.model, opts)
@@ -436,10 +436,10 @@ def test_builder_matches_oracle_setup():
 
 
 def test_dt_and_step_count_closed_forms():
-    """config0: dt = 0.95 h^2 / (6 sigma_max) = 0.95*36/(6*220000) ms; the
+    """lattice dyad at 6 nm: dt = 0.95 h^2 / (6 sigma_max) = 0.95*36/(6*220000) ms; the
     reference loop (t accumulated in double, calcium_sparks.c:190) runs
     ceil(10 ms / dt) = 385965 steps, counted here by an IEEE-double loop."""
-    d = dyad.config0()
+    d = dyad.lattice_dyad()
     s = cs.Setup(d.geometry, d.model, cs.Options(seed=1))
     assert s.dt == pytest.approx(0.95 * 36 / (6 * 220000), rel=1e-15)
     t, n = 0.0, 0
