@@ -47,7 +47,7 @@ constexpr int BX = 64;          /* one wavefront along x                  */
 constexpr int GATE_THREADS = 256;
 constexpr int FLUX_THREADS = 256;
 constexpr int STAT_THREADS = 256;
-constexpr int STAT_BLOCKS = 512;
+constexpr int STAT_BLOCKS = 1024;
 
 thread_local std::string g_err;
 
@@ -106,10 +106,21 @@ struct DevParams {
   FluxParams flux[MAXF];
 };
 
+/* Grid.  Fields are stored [z][species][y][x]: the ns species planes of
+ * one z lie next to each other, so a pass over the fields is one read and
+ * one write stream instead of one per species. */
 struct Geo {
   int nx, ny, nz;
+  uint32_t ns;
   uint64_t plane, nv;
 };
+
+/* element index of species s at voxel v = z*plane + (y*nx + x) */
+__host__ __device__ inline uint64_t elem(const Geo& g, uint32_t s, uint64_t v)
+{
+  const uint64_t z = v / g.plane;
+  return (z * g.ns + s) * g.plane + (v - z * g.plane);
+}
 
 /* ------------------------------------------------------------ kernels */
 
@@ -126,6 +137,59 @@ struct LaunchCfg {
   int yb;        /* >0: row-blocked kernel, yb rows per lane and `rows`
                     wavefronts side by side along x per block              */
 };
+
+/* Cross-lane moves within one 64-wide wavefront (DPP wave_shr:1 and
+ * wave_shl:1).  from_left: lane i receives lane i-1's value and lane 0
+ * keeps `edge`; from_right: lane i receives lane i+1's value and lane 63
+ * keeps `edge`.  All 64 lanes must be active. */
+__device__ inline float dpp_from_left(float v, float edge)
+{
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+__device__ inline float dpp_from_right(float v, float edge)
+{
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x130, 0xF, 0xF, false));
+}
+__device__ inline double dpp_from_left(double v, double edge)
+{
+  const uint64_t a = (uint64_t)__double_as_longlong(v), e = (uint64_t)__double_as_longlong(edge);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)e, (int)(uint32_t)a, 0x138, 0xF, 0xF, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)(e >> 32), (int)(uint32_t)(a >> 32), 0x138, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ inline double dpp_from_right(double v, double edge)
+{
+  const uint64_t a = (uint64_t)__double_as_longlong(v), e = (uint64_t)__double_as_longlong(edge);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)e, (int)(uint32_t)a, 0x130, 0xF, 0xF, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)(e >> 32), (int)(uint32_t)(a >> 32), 0x130, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+/* Tile handled by hardware block b.  Blocks are dispatched to the 8 XCDs
+ * round-robin (b % 8).  xcd 0 keeps that order; xcd 1 gives XCD k the
+ * contiguous tiles [k*per, (k+1)*per); xcd = G >= 2 hands out groups of G
+ * consecutive tiles round-robin, so each group shares one XCD's L2 while
+ * the groups spread over all XCDs.  The launch covers a multiple of 8
+ * (xcd 1) or 8*G (xcd G) blocks, so the map is a permutation; tiles past
+ * nblocks exit. */
+__device__ inline int xcd_tile(int b, int nblocks, int xcd)
+{
+  if (xcd == 0) return b;
+  if (xcd == 1) {
+    const int per = (nblocks + 7) / 8;
+    return (b % 8) * per + b / 8;
+  }
+  const int k = b % 8, j = b / 8;
+  return ((j / xcd) * 8 + k) * xcd + j % xcd;
+}
+
+/* Launch size for nblocks tiles under xcd_tile's map. */
+inline int xcd_launch(int nblocks, int xcd)
+{
+  if (xcd == 0) return nblocks;
+  const int m = xcd == 1 ? 8 : 8 * xcd;
+  return (nblocks + m - 1) / m * m;
+}
 
 /* Per-block copies of the tables every voxel update reads. */
 template <typename T, int NS>
@@ -304,10 +368,7 @@ k_stencil(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict
    * over the 8 XCDs hands each XCD a contiguous run of tiles, so halo rows
    * shared by neighbouring tiles hit in that XCD's L2 */
   int bid = blockIdx.x;
-  if (xcd) {
-    const int per = (nblocks + 7) / 8;
-    bid = (bid % 8) * per + bid / 8;
-  }
+  bid = xcd_tile(bid, nblocks, xcd);
   if (bid >= nblocks) return;
   const int bx = bid % gx, by = (bid / gx) % gy, bz = bid / (gx * gy);
 
@@ -350,17 +411,25 @@ k_stencil(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict
   const bool xyface = x == 0 || y == 0 || x + 1 == g.nx || y + 1 == g.ny;
   const int ztop_plane = g.nz - 1;
   const size_t plane = g.plane;
+  /* x neighbours come from the adjacent lane (DPP): lanes hold consecutive
+   * cells of a row in both tilings.  Lane 0 loads its x-1 and lane 63 its
+   * x+1, which lie outside the wavefront; at a row end the neighbour is
+   * the voxel itself.  Lanes that left early (past the row or the plane)
+   * are never read, since their left neighbour sits at x = nx-1. */
+  const int lane = threadIdx.x;
+  const bool has_xm = x > 0, has_xp = x + 1 < g.nx;
+  const bool edge_lane = (lane == 0 && has_xm) || (lane == BX - 1 && has_xp);
+  const uint32_t bedge = lane == 0 ? bxm : bxp;
 
-  /* descriptors over the planes this block touches: [zb, ze) of every
-   * species of u and of the ids, [z0, z1) of the next state */
+  /* descriptors over the z-groups this block touches: [zb, ze) of u and
+   * of the ids, [z0, z1) of the next state; species s of group z sits at
+   * byte (z - base) * gbytes + s * pbytes (the host keeps (zchunk + 3)
+   * groups below 2^32 bytes) */
   const int zb = max(z0 - 1, 0), ze = min(g.nz, z1 + 2);
   const uint32_t pbytes = (uint32_t)plane * ES;
-  Rsrc rin[NS], rout[NS];
-#pragma unroll
-  for (int s = 0; s < NS; s++) {
-    rin[s] = make_rsrc(u + s * g.nv + (size_t)zb * plane, (uint32_t)(ze - zb) * pbytes);
-    rout[s] = make_rsrc(un + s * g.nv + (size_t)z0 * plane, (uint32_t)(z1 - z0) * pbytes);
-  }
+  const uint32_t gbytes = (uint32_t)NS * pbytes;
+  const Rsrc rin = make_rsrc(u + (size_t)zb * NS * plane, (uint32_t)(ze - zb) * gbytes);
+  const Rsrc rout = make_rsrc(un + (size_t)z0 * NS * plane, (uint32_t)(z1 - z0) * gbytes);
   const Rsrc rid = make_rsrc(ids + (size_t)zb * plane, (uint32_t)(ze - zb) * (uint32_t)plane);
 
   /* One plane: um, uc, up hold planes z-1, z, z+1 (indices clamped, so a
@@ -369,37 +438,44 @@ k_stencil(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict
    * after this plane's neighbours, so the wait for the neighbours (mostly
    * cache hits: plane z arrived a plane ago) leaves uq in flight. */
   auto plane_step = [&](const int z, const T (&um)[NS], const T (&uc)[NS], T (&up)[NS], T (&uq)[NS]) {
-    const uint32_t sz = (uint32_t)(z - zb) * pbytes;           /* plane z in rin */
+    const uint32_t sz = (uint32_t)(z - zb) * gbytes;           /* group z in rin */
     const uint32_t szi = (uint32_t)(z - zb) * (uint32_t)plane; /* plane z in rid */
     const bool ztop = z < ztop_plane;
     if (PF == 1) {
-      const uint32_t sp = (uint32_t)(min(z + 1, ztop_plane) - zb) * pbytes;
+      const uint32_t sp = (uint32_t)(min(z + 1, ztop_plane) - zb) * gbytes;
 #pragma unroll
-      for (int s = 0; s < NS; s++) up[s] = bload<T>(rin[s], bc, sp);
+      for (int s = 0; s < NS; s++) up[s] = bload<T>(rin, bc, sp + s * pbytes);
     }
     /* the in-plane neighbours of every diffusing species are loaded before
      * any of them is used, so one wait covers them all */
     const uint32_t id = bload_u8(rid, col, szi);
-    T nxm[NS], nxp[NS], nym[NS], nyp[NS];
+    T nxm[NS], nxp[NS], nym[NS], nyp[NS], nxe[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++)
       if ((dmask >> s) & 1u) {
-        nxm[s] = bload<T>(rin[s], bxm, sz);
-        nxp[s] = bload<T>(rin[s], bxp, sz);
-        nym[s] = bload<T>(rin[s], bym, sz);
-        nyp[s] = bload<T>(rin[s], byp, sz);
+        nym[s] = bload<T>(rin, bym, sz + s * pbytes);
+        nyp[s] = bload<T>(rin, byp, sz + s * pbytes);
+        nxe[s] = uc[s];
+        if (edge_lane) nxe[s] = bload<T>(rin, bedge, sz + s * pbytes);
       }
     if (PF == 2) {
       asm volatile("" ::: "memory");   /* keep the plane-ahead loads last */
-      const uint32_t sq = (uint32_t)(min(z + 2, ztop_plane) - zb) * pbytes;
+      const uint32_t sq = (uint32_t)(min(z + 2, ztop_plane) - zb) * gbytes;
 #pragma unroll
-      for (int s = 0; s < NS; s++) uq[s] = bload<T>(rin[s], bc, sq);
+      for (int s = 0; s < NS; s++) uq[s] = bload<T>(rin, bc, sq + s * pbytes);
     } else {
       /* the next plane is complete before the first store, so only stores
        * are outstanding between planes */
 #pragma unroll
       for (int s = 0; s < NS; s++) asm volatile("" ::"v"(up[s]));
     }
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+      if ((dmask >> s) & 1u) {
+        const T l = dpp_from_left(uc[s], nxe[s]), r = dpp_from_right(uc[s], nxe[s]);
+        nxm[s] = has_xm ? l : uc[s];
+        nxp[s] = has_xp ? r : uc[s];
+      }
     const int d = id & 7;
     const bool uni = (id & 0x80) != 0;
     T nw[NS];
@@ -451,9 +527,9 @@ k_stencil(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict
       for (int s = 0; s < NS; s++)
         if ((fm >> s) & 1u) nw[s] = S.init[d][s];
     }
-    const uint32_t so = (uint32_t)(z - z0) * pbytes;
+    const uint32_t so = (uint32_t)(z - z0) * gbytes;
 #pragma unroll
-    for (int s = 0; s < NS; s++) bstore<T>(rout[s], bc, so, nw[s], NT);
+    for (int s = 0; s < NS; s++) bstore<T>(rout, bc, so + s * pbytes, nw[s], NT);
   };
 
   /* register ring of PF + 2 planes, rotated by renaming: the loop body
@@ -462,9 +538,9 @@ k_stencil(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict
   T ra[NS], rb[NS], rc[NS], rd[NS];
 #pragma unroll
   for (int s = 0; s < NS; s++) {
-    ra[s] = bload<T>(rin[s], bc, (uint32_t)(max(z0 - 1, 0) - zb) * pbytes);
-    rb[s] = bload<T>(rin[s], bc, (uint32_t)(z0 - zb) * pbytes);
-    if (PF == 2) rc[s] = bload<T>(rin[s], bc, (uint32_t)(min(z0 + 1, ztop_plane) - zb) * pbytes);
+    ra[s] = bload<T>(rin, bc, (uint32_t)(max(z0 - 1, 0) - zb) * gbytes + s * pbytes);
+    rb[s] = bload<T>(rin, bc, (uint32_t)(z0 - zb) * gbytes + s * pbytes);
+    if (PF == 2) rc[s] = bload<T>(rin, bc, (uint32_t)(min(z0 + 1, ztop_plane) - zb) * gbytes + s * pbytes);
   }
   /* drain the prologue loads, so the loop header's wait does not also
    * cover the previous plane's stores (stores count in vmcnt on gfx9) */
@@ -503,10 +579,7 @@ k_stencil_lds(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __rest
   __shared__ BlockTables<T, NS> S;
 
   int bid = blockIdx.x;
-  if (xcd) {
-    const int per = (nblocks + 7) / 8;
-    bid = (bid % 8) * per + bid / 8;
-  }
+  bid = xcd_tile(bid, nblocks, xcd);
   if (bid >= nblocks) return;
   const int rows = (int)blockDim.y;
   const int bx = bid % gx, by = (bid / gx) % gy, bz = bid / (gx * gy);
@@ -552,14 +625,15 @@ k_stencil_lds(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __rest
   T um[NS], uc[NS], up[NS];
   T hxv[NS], hyv[NS], hy2v[NS];
   {
-    const uint64_t pc = (uint64_t)z0 * g.plane, pm = (uint64_t)max(z0 - 1, 0) * g.plane;
+    /* z-group offsets: species s of plane z starts at (z*NS + s)*plane */
+    const uint64_t pc = (uint64_t)z0 * NS * g.plane, pm = (uint64_t)max(z0 - 1, 0) * NS * g.plane;
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-      uc[s] = u[s * g.nv + pc + col];
-      um[s] = u[s * g.nv + pm + col];
-      if (hx) hxv[s] = u[s * g.nv + pc + hxcol];
-      if (hy) hyv[s] = u[s * g.nv + pc + hycol];
-      if (hy2) hy2v[s] = u[s * g.nv + pc + hy2col];
+      uc[s] = u[pc + s * g.plane + col];
+      um[s] = u[pm + s * g.plane + col];
+      if (hx) hxv[s] = u[pc + s * g.plane + hxcol];
+      if (hy) hyv[s] = u[pc + s * g.plane + hycol];
+      if (hy2) hy2v[s] = u[pc + s * g.plane + hy2col];
     }
   }
   int buf = 0;
@@ -578,14 +652,14 @@ k_stencil_lds(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __rest
     }
     /* prefetch plane z+1 (clamped) */
     {
-      const uint64_t pp = (uint64_t)min(z + 1, ztop_plane) * g.plane;
+      const uint64_t pp = (uint64_t)min(z + 1, ztop_plane) * NS * g.plane;
 #pragma unroll
       for (int s = 0; s < NS; s++) {
-        up[s] = u[s * g.nv + pp + col];
+        up[s] = u[pp + s * g.plane + col];
         if (z + 1 < z1) {
-          if (hx) hxv[s] = u[s * g.nv + pp + hxcol];
-          if (hy) hyv[s] = u[s * g.nv + pp + hycol];
-          if (hy2) hy2v[s] = u[s * g.nv + pp + hy2col];
+          if (hx) hxv[s] = u[pp + s * g.plane + hxcol];
+          if (hy) hyv[s] = u[pp + s * g.plane + hycol];
+          if (hy2) hy2v[s] = u[pp + s * g.plane + hy2col];
         }
       }
     }
@@ -633,7 +707,7 @@ k_stencil_lds(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __rest
           if ((fm >> s) & 1u) nw[s] = S.init[d][s];
       }
 #pragma unroll
-      for (int s = 0; s < NS; s++) store_next(un + s * g.nv + v, nw[s], NT);
+      for (int s = 0; s < NS; s++) store_next(un + ((uint64_t)z * NS + s) * g.plane + col, nw[s], NT);
     }
 #pragma unroll
     for (int s = 0; s < NS; s++) {
@@ -642,33 +716,6 @@ k_stencil_lds(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __rest
     }
     buf ^= 1;
   }
-}
-
-/* Cross-lane moves within one 64-wide wavefront (DPP wave_shr:1 and
- * wave_shl:1).  from_left: lane i receives lane i-1's value and lane 0
- * keeps `edge`; from_right: lane i receives lane i+1's value and lane 63
- * keeps `edge`.  All 64 lanes must be active. */
-__device__ inline float dpp_from_left(float v, float edge)
-{
-  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x138, 0xF, 0xF, false));
-}
-__device__ inline float dpp_from_right(float v, float edge)
-{
-  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x130, 0xF, 0xF, false));
-}
-__device__ inline double dpp_from_left(double v, double edge)
-{
-  const uint64_t a = (uint64_t)__double_as_longlong(v), e = (uint64_t)__double_as_longlong(edge);
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)e, (int)(uint32_t)a, 0x138, 0xF, 0xF, false);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)(e >> 32), (int)(uint32_t)(a >> 32), 0x138, 0xF, 0xF, false);
-  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-}
-__device__ inline double dpp_from_right(double v, double edge)
-{
-  const uint64_t a = (uint64_t)__double_as_longlong(v), e = (uint64_t)__double_as_longlong(edge);
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)e, (int)(uint32_t)a, 0x130, 0xF, 0xF, false);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)(e >> 32), (int)(uint32_t)(a >> 32), 0x130, 0xF, 0xF, false);
-  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
 /* Row-blocked variant.  A wavefront owns 64 consecutive x of Y consecutive
@@ -692,10 +739,7 @@ k_stencil_rows(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __res
   __shared__ BlockTables<T, NS> S;
 
   int bid = blockIdx.x;
-  if (xcd) {
-    const int per = (nblocks + 7) / 8;
-    bid = (bid % 8) * per + bid / 8;
-  }
+  bid = xcd_tile(bid, nblocks, xcd);
   if (bid >= nblocks) return;
   const int nthr = BX * blockDim.y;
   const int tid = threadIdx.y * BX + threadIdx.x;
@@ -736,12 +780,9 @@ k_stencil_rows(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __res
   const int zb = max(z0 - 1, 0), ze = min(g.nz, z1 + 2);
   const uint32_t ES = (uint32_t)sizeof(T);
   const uint32_t pbytes = (uint32_t)pstride * ES;
-  Rsrc rin[NS], rout[NS];
-#pragma unroll
-  for (int s = 0; s < NS; s++) {
-    rin[s] = make_rsrc(u + s * g.nv + (size_t)zb * pstride, (uint32_t)(ze - zb) * pbytes);
-    rout[s] = make_rsrc(un + s * g.nv + (size_t)z0 * pstride, (uint32_t)(z1 - z0) * pbytes);
-  }
+  const uint32_t gbytes = (uint32_t)NS * pbytes;   /* one z-group of all species */
+  const Rsrc rin = make_rsrc(u + (size_t)zb * NS * pstride, (uint32_t)(ze - zb) * gbytes);
+  const Rsrc rout = make_rsrc(un + (size_t)z0 * NS * pstride, (uint32_t)(z1 - z0) * gbytes);
   const Rsrc rid = make_rsrc(ids + (size_t)zb * pstride, (uint32_t)(ze - zb) * (uint32_t)pstride);
   uint32_t vox[Y];    /* voxel index of (x, row r) within a plane */
 #pragma unroll
@@ -753,9 +794,9 @@ k_stencil_rows(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __res
   for (int s = 0; s < NS; s++)
 #pragma unroll
     for (int r = 0; r < Y; r++) {
-      cc[s][r] = bload<T>(rin[s], roff[r], (uint32_t)(z0 - zb) * pbytes);
-      cm[s][r] = bload<T>(rin[s], roff[r], (uint32_t)(max(z0 - 1, 0) - zb) * pbytes);
-      if (PF == 2) cp[s][r] = bload<T>(rin[s], roff[r], (uint32_t)(min(z0 + 1, ztop_plane) - zb) * pbytes);
+      cc[s][r] = bload<T>(rin, roff[r], (uint32_t)(z0 - zb) * gbytes + s * pbytes);
+      cm[s][r] = bload<T>(rin, roff[r], (uint32_t)(max(z0 - 1, 0) - zb) * gbytes + s * pbytes);
+      if (PF == 2) cp[s][r] = bload<T>(rin, roff[r], (uint32_t)(min(z0 + 1, ztop_plane) - zb) * gbytes + s * pbytes);
     }
   /* Drain the prologue loads here.  Otherwise the wait the compiler places
    * at the loop header for them also covers the previous plane's stores
@@ -763,7 +804,7 @@ k_stencil_rows(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __res
    * stores to complete. */
   __builtin_amdgcn_s_waitcnt(0);
   for (int z = z0; z < z1; z++) {
-    const uint32_t sz = (uint32_t)(z - zb) * pbytes;
+    const uint32_t sz = (uint32_t)(z - zb) * gbytes;
     const uint32_t szi = (uint32_t)(z - zb) * (uint32_t)pstride;
     /* this plane's y halos and x edges first, then the planes ahead, so a
      * wait for the former leaves the latter in flight */
@@ -775,11 +816,11 @@ k_stencil_rows(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __res
 #pragma unroll
       for (int r = 0; r < Y; r++) ex[s][r] = cc[s][r];
       if ((dmask >> s) & 1u) {
-        hm[s] = bload<T>(rin[s], hmoff, sz);
-        hp[s] = bload<T>(rin[s], hpoff, sz);
+        hm[s] = bload<T>(rin, hmoff, sz + s * pbytes);
+        hp[s] = bload<T>(rin, hpoff, sz + s * pbytes);
         if (edge_lane) {
 #pragma unroll
-          for (int r = 0; r < Y; r++) ex[s][r] = bload<T>(rin[s], (uint32_t)((int32_t)roff[r] + edelta), sz);
+          for (int r = 0; r < Y; r++) ex[s][r] = bload<T>(rin, (uint32_t)((int32_t)roff[r] + edelta), sz + s * pbytes);
         }
       }
     }
@@ -787,17 +828,17 @@ k_stencil_rows(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __res
 #pragma unroll
     for (int r = 0; r < Y; r++) idw[r] = bload_u8(rid, vox[r], szi);
     if (PF == 1) {
-      const uint32_t sp = (uint32_t)(min(z + 1, ztop_plane) - zb) * pbytes;
+      const uint32_t sp = (uint32_t)(min(z + 1, ztop_plane) - zb) * gbytes;
 #pragma unroll
       for (int s = 0; s < NS; s++)
 #pragma unroll
-        for (int r = 0; r < Y; r++) cp[s][r] = bload<T>(rin[s], roff[r], sp);
+        for (int r = 0; r < Y; r++) cp[s][r] = bload<T>(rin, roff[r], sp + s * pbytes);
     } else if (z + 1 < z1) {
-      const uint32_t sq = (uint32_t)(min(z + 2, ztop_plane) - zb) * pbytes;
+      const uint32_t sq = (uint32_t)(min(z + 2, ztop_plane) - zb) * gbytes;
 #pragma unroll
       for (int s = 0; s < NS; s++)
 #pragma unroll
-        for (int r = 0; r < Y; r++) cq[s][r] = bload<T>(rin[s], roff[r], sq);
+        for (int r = 0; r < Y; r++) cq[s][r] = bload<T>(rin, roff[r], sq + s * pbytes);
     }
 
     /* x neighbours of every row through DPP (all lanes active here) */
@@ -869,7 +910,7 @@ k_stencil_rows(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __res
       }
       if (xin && yr < g.ny) {
 #pragma unroll
-        for (int s = 0; s < NS; s++) bstore<T>(rout[s], roff[r], (uint32_t)(z - z0) * pbytes, nw[s], NT);
+        for (int s = 0; s < NS; s++) bstore<T>(rout, roff[r], (uint32_t)(z - z0) * gbytes + s * pbytes, nw[s], NT);
       }
     }
 #pragma unroll
@@ -888,7 +929,7 @@ k_stencil_rows(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __res
  * voxel (species.c:2569). */
 template <typename T>
 __global__ void __launch_bounds__(GATE_THREADS)
-k_gate(const T* __restrict__ u, const uint32_t* __restrict__ pairs, uint8_t* __restrict__ states,
+k_gate(const T* __restrict__ u, Geo g, const uint32_t* __restrict__ pairs, uint8_t* __restrict__ states,
        const DevParams<T>* __restrict__ P, uint64_t ti_arg, const uint64_t* __restrict__ ti_ptr,
        uint32_t D, uint32_t seed, uint32_t instance, const double* __restrict__ uniforms,
        uint64_t uniform_base)
@@ -906,7 +947,7 @@ k_gate(const T* __restrict__ u, const uint32_t* __restrict__ pairs, uint8_t* __r
         if (p[8] <= t + dts) st = 0;
       } else {
         const uint32_t to = pairs[2 * (F.pair_off + ch) + 1];
-        const double c0 = (double)u[to];
+        const double c0 = (double)u[elem(g, 0, to)];
         const double kc = cs_rate_close(c0, p);
         const double ko = cs_rate_open(c0, p);
         const double r = uniforms ? uniforms[uniform_base + F.uniform_off + ch]
@@ -941,7 +982,6 @@ k_flux(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__ 
         z + 1 == (uint32_t)g.nz)
       return;
   }
-  const T* us = u + s * g.nv;
   const double dt = P->dt, h = P->h;
   double sum = 0.0;
   bool any = false;
@@ -952,7 +992,7 @@ k_flux(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__ 
     const uint32_t p = ent_pair[e];
     if (F.discrete && !states[F.state_off + p]) continue;
     const uint32_t from = pairs[2 * (F.pair_off + p)], to = pairs[2 * (F.pair_off + p) + 1];
-    const T u0 = us[from], u1 = us[to];
+    const T u0 = u[elem(g, s, from)], u1 = u[elem(g, s, to)];
     double J;
     if (F.kind == CS_FLUX_RYR)
       J = (double)((u0 - u1) * (T)F.half_1mK0);
@@ -962,7 +1002,7 @@ k_flux(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__ 
     any = true;
   }
   if (any) {
-    T* o = un + s * g.nv + v;
+    T* o = un + elem(g, s, v);
     *o = (T)((double)*o + sum);
   }
 }
@@ -995,24 +1035,31 @@ __global__ void __launch_bounds__(STAT_THREADS)
 k_stats(const T* __restrict__ u, const uint8_t* __restrict__ ids, Geo g, uint32_t ns, double dV,
         double* __restrict__ partial /* [grid][ns][MAXD][3] */)
 {
+  /* block (bx, by) takes planes by, by + gridDim.y, ... and within each the
+   * columns bx*blockDim + t strided by the grid width: a fixed assignment,
+   * so the FP64 sums do not depend on scheduling */
   __shared__ double sh[3 * MAXD * (STAT_THREADS / 64)];
   const int nw = STAT_THREADS / 64;
+  const uint32_t blk = blockIdx.y * gridDim.x + blockIdx.x;
   for (uint32_t s = 0; s < ns; s++) {
     double mn[MAXD], mx[MAXD], sm[MAXD];
 #pragma unroll
     for (int d = 0; d < MAXD; d++) { mn[d] = FLT_MAX; mx[d] = FLT_MIN; sm[d] = 0.0; }
-    const T* us = u + s * g.nv;
-    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.nv;
-         v += (uint64_t)gridDim.x * blockDim.x) {
-      const int dv = ids[v] & 7;
-      const double val = (double)us[v];
+    for (int z = blockIdx.y; z < g.nz; z += gridDim.y) {
+      const T* us = u + ((uint64_t)z * ns + s) * g.plane;
+      const uint8_t* iz = ids + (uint64_t)z * g.plane;
+      for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < g.plane;
+           c += (uint64_t)gridDim.x * blockDim.x) {
+        const int dv = iz[c] & 7;
+        const double val = (double)us[c];
 #pragma unroll
-      for (int d = 0; d < MAXD; d++)
-        if (d == dv) {
-          mn[d] = val < mn[d] ? val : mn[d];
-          mx[d] = val > mx[d] ? val : mx[d];
-          sm[d] += val * dV;
-        }
+        for (int d = 0; d < MAXD; d++)
+          if (d == dv) {
+            mn[d] = val < mn[d] ? val : mn[d];
+            mx[d] = val > mx[d] ? val : mx[d];
+            sm[d] += val * dV;
+          }
+      }
     }
     block_reduce3<T>(mn, mx, sm, sh);
     __syncthreads();
@@ -1024,26 +1071,39 @@ k_stats(const T* __restrict__ u, const uint8_t* __restrict__ ids, Geo g, uint32_
         b = fmax(b, sh[(1 * MAXD + d) * nw + w]);
         c += sh[(2 * MAXD + d) * nw + w];
       }
-      double* o = partial + (((uint64_t)blockIdx.x * ns + s) * MAXD + d) * 3;
+      double* o = partial + (((uint64_t)blk * ns + s) * MAXD + d) * 3;
       o[0] = a; o[1] = b; o[2] = c;
     }
     __syncthreads();
   }
 }
 
-__global__ void k_stats_final(const double* __restrict__ partial, uint32_t nblocks, uint32_t ns,
-                              double* __restrict__ out /* [ns][MAXD][3] */)
+/* One block per (species, domain) entry: threads stride over the block
+ * partials, then a fixed-order tree in LDS combines them. */
+__global__ void __launch_bounds__(STAT_THREADS)
+k_stats_final(const double* __restrict__ partial, uint32_t nblocks, uint32_t ns,
+              double* __restrict__ out /* [ns][MAXD][3] */)
 {
-  const uint32_t i = threadIdx.x;
-  if (i >= ns * MAXD) return;
+  __shared__ double sa[STAT_THREADS], sb[STAT_THREADS], sc[STAT_THREADS];
+  const uint32_t i = blockIdx.x;   /* s * MAXD + d */
   double a = FLT_MAX, b = FLT_MIN, c = 0.0;
-  for (uint32_t k = 0; k < nblocks; k++) {
+  for (uint32_t k = threadIdx.x; k < nblocks; k += blockDim.x) {
     const double* p = partial + ((uint64_t)k * ns * MAXD + i) * 3;
     a = p[0] < a ? p[0] : a;
     b = p[1] > b ? p[1] : b;
     c += p[2];
   }
-  out[3 * i + 0] = a; out[3 * i + 1] = b; out[3 * i + 2] = c;
+  sa[threadIdx.x] = a; sb[threadIdx.x] = b; sc[threadIdx.x] = c;
+  __syncthreads();
+  for (uint32_t w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      sa[threadIdx.x] = fmin(sa[threadIdx.x], sa[threadIdx.x + w]);
+      sb[threadIdx.x] = fmax(sb[threadIdx.x], sb[threadIdx.x + w]);
+      sc[threadIdx.x] += sc[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { out[3 * i + 0] = sa[0]; out[3 * i + 1] = sb[0]; out[3 * i + 2] = sc[0]; }
 }
 
 /* Linescan reductions (species.c:1960-2035).  The reference's 3-D
@@ -1056,16 +1116,18 @@ __global__ void k_stats_final(const double* __restrict__ partial, uint32_t nbloc
  * wavefront per (y, z) row, butterfly reduction across the wavefront. */
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_linescan_rows(const T* __restrict__ u, const uint8_t* __restrict__ ids, Geo g, uint32_t dmask,
+k_linescan_rows(const T* __restrict__ u, uint32_t s, const uint8_t* __restrict__ ids, Geo g, uint32_t dmask,
                 const double* __restrict__ wx, double* __restrict__ rows)
 {
   const uint64_t row = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
   if (row >= (uint64_t)g.ny * (uint64_t)g.nz) return;   /* whole wavefront */
   const int lane = threadIdx.x & 63;
-  const uint64_t base = row * (uint64_t)g.nx;
+  const uint64_t z = row / g.ny, y = row - z * g.ny;
+  const uint64_t ib = row * (uint64_t)g.nx;                                 /* ids   */
+  const uint64_t ub = ((z * g.ns + s) * g.ny + y) * (uint64_t)g.nx;       /* field */
   double acc = 0.0;
   for (int x = lane; x < g.nx; x += 64)
-    if ((dmask >> (ids[base + x] & 7)) & 1u) acc += (double)u[base + x] * wx[x];
+    if ((dmask >> (ids[ib + x] & 7)) & 1u) acc += (double)u[ub + x] * wx[x];
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
   if (lane == 0) rows[row] = acc;
 }
@@ -1075,16 +1137,17 @@ k_linescan_rows(const T* __restrict__ u, const uint8_t* __restrict__ ids, Geo g,
  * lane per x. */
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_linescan_cols(const T* __restrict__ u, const uint8_t* __restrict__ ids, Geo g, uint32_t dmask,
+k_linescan_cols(const T* __restrict__ u, uint32_t s, const uint8_t* __restrict__ ids, Geo g, uint32_t dmask,
                 const double* __restrict__ wy, int ychunk, double* __restrict__ part)
 {
   const int z = blockIdx.x, c = blockIdx.y;
   const int y0 = c * ychunk, y1 = min(g.ny, y0 + ychunk);
+  const T* us = u + ((uint64_t)z * g.ns + s) * g.plane;
   for (int x = threadIdx.x; x < g.nx; x += blockDim.x) {
     double acc = 0.0;
     for (int y = y0; y < y1; y++) {
       const uint64_t v = ((uint64_t)z * g.ny + y) * g.nx + x;
-      if ((dmask >> (ids[v] & 7)) & 1u) acc += (double)u[v] * wy[y];
+      if ((dmask >> (ids[v] & 7)) & 1u) acc += (double)us[(uint64_t)y * g.nx + x] * wy[y];
     }
     part[((uint64_t)z * gridDim.y + c) * g.nx + x] = acc;
   }
@@ -1107,7 +1170,7 @@ __global__ void k_fill_init(T* __restrict__ u, const uint8_t* __restrict__ ids,
   for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.nv;
        v += (uint64_t)gridDim.x * blockDim.x) {
     const int d = ids[v] & 7;
-    for (uint32_t s = 0; s < ns; s++) u[s * g.nv + v] = P->init[d][s];
+    for (uint32_t s = 0; s < ns; s++) u[elem(g, s, v)] = P->init[d][s];
   }
 }
 
@@ -1167,13 +1230,22 @@ size_t lds_tile_bytes(size_t esize, uint32_t ns, int rows)
 }
 constexpr size_t LDS_TILE_LIMIT = 60 * 1024;
 
-/* Default launch shape, the fastest measured on MI355X for the 32M-voxel
- * single dyad in FP32: linear columns, blocks of 8 wavefronts, 12 planes
- * per block, one plane ahead, non-temporal stores of the next state. */
-LaunchCfg default_launch(const Geo& g)
+/* Largest z chunk whose planes (chunk + 3 z-groups) fit the 32-bit
+ * offsets of the stencil kernels' buffer descriptors. */
+int max_zchunk(const Geo& g, size_t esize)
 {
-  LaunchCfg L{8, 12, 0, 1, 1, 0, 1, 0};
-  L.zchunk = std::min(L.zchunk, g.nz);
+  const uint64_t group = g.plane * esize * g.ns;
+  return (int)std::min<uint64_t>((uint64_t)g.nz, ((1ull << 32) - 1) / group - 3);
+}
+
+/* Default launch shape, the fastest measured on MI355X for the 32M-voxel
+ * single dyad in FP32: linear columns, blocks of 4 wavefronts, 8 planes
+ * per block, groups of 16 tiles per XCD, one plane ahead, non-temporal
+ * stores of the next state. */
+LaunchCfg default_launch(const Geo& g, size_t esize)
+{
+  LaunchCfg L{4, 8, 16, 1, 1, 0, 1, 0};
+  L.zchunk = std::min(L.zchunk, max_zchunk(g, esize));
   return L;
 }
 
@@ -1283,10 +1355,13 @@ int validate(const cs_problem* p)
     return set_err(CS_EINVAL, "num_domains %u outside 1..%d", p->num_domains, MAXD);
   if (p->precision != CS_PREC_F64 && p->precision != CS_PREC_F32)
     return set_err(CS_EINVAL, "unknown precision %u", p->precision);
-  /* the stencil kernels address a plane with 32-bit byte offsets */
-  const uint64_t plane_bytes = (uint64_t)p->n[0] * p->n[1] * (p->precision == CS_PREC_F64 ? 8 : 4);
-  if (plane_bytes >= (1ull << 31)) return set_err(CS_ERANGE, "an x-y plane of %llu bytes exceeds 2^31",
-                                                 (unsigned long long)plane_bytes);
+  /* the stencil kernels address a block's z-groups (every species of a
+   * plane) with 32-bit byte offsets: at least 4 groups must fit */
+  const uint64_t group_bytes = (uint64_t)p->n[0] * p->n[1] * (p->precision == CS_PREC_F64 ? 8 : 4) *
+                               (p->num_species ? p->num_species : 1);
+  if (4 * group_bytes >= (1ull << 32))
+    return set_err(CS_ERANGE, "an x-y plane of all species (%llu bytes) exceeds 2^30",
+                   (unsigned long long)group_bytes);
   if (!p->domain_ids) return set_err(CS_EINVAL, "domain_ids is NULL");
   if (p->num_buffers > MAXS) return set_err(CS_EINVAL, "too many buffers");
   if (p->num_fluxes > MAXF) return set_err(CS_EINVAL, "too many fluxes");
@@ -1358,7 +1433,7 @@ void launch_stencil_v(cs_context* c, const LaunchCfg& L, const void* uin, void* 
     gx = (g.nx + BX * L.rows - 1) / (BX * L.rows);
     gy = (g.ny + L.yb - 1) / L.yb;
     const int nblocks = gx * gy * gz;
-    const int launch = L.xcd ? ((nblocks + 7) / 8) * 8 : nblocks;
+    const int launch = xcd_launch(nblocks, L.xcd);
     if (L.yb == 4)
       hipLaunchKernelGGL((k_stencil_rows<T, NS, 4, PF, NT>), dim3(launch), dim3(BX, L.rows), 0, c->stream,
                          (const T*)uin, (T*)uout, c->d_ids, (const DevParams<T>*)c->d_params, g, dmask,
@@ -1377,7 +1452,7 @@ void launch_stencil_v(cs_context* c, const LaunchCfg& L, const void* uin, void* 
     gy = (g.ny + L.rows - 1) / L.rows;
   }
   const int nblocks = gx * gy * gz;
-  const int launch = L.xcd ? ((nblocks + 7) / 8) * 8 : nblocks;
+  const int launch = xcd_launch(nblocks, L.xcd);
   if (L.lds) {
     const size_t lds = sizeof(T) * 2 * NS * (L.rows + 2) * (BX + 2);
     hipLaunchKernelGGL((k_stencil_lds<T, NS, NT>), dim3(launch), dim3(BX, L.rows), lds, c->stream,
@@ -1448,7 +1523,7 @@ int step_one(cs_context* c, uint64_t ti, const double* d_uni, uint64_t uni_base,
     if (ti % c->flux_substep[f] == 0) fluxmask |= 1u << f;
   const T* ucur = (const T*)c->d_u[c->cur];
   if (c->ndiscrete && ti % p.stochastic_substep == 0) {
-    hipLaunchKernelGGL(k_gate<T>, dim3(1), dim3(GATE_THREADS), 0, c->stream, ucur, c->d_pairs,
+    hipLaunchKernelGGL(k_gate<T>, dim3(1), dim3(GATE_THREADS), 0, c->stream, ucur, c->geo, c->d_pairs,
                        c->d_states, (const DevParams<T>*)c->d_params, ti, (const uint64_t*)nullptr,
                        p.stochastic_substep, (uint32_t)p.seed, (uint32_t)p.instance, d_uni, uni_base);
   }
@@ -1653,6 +1728,7 @@ int cs_create(const cs_problem* p, cs_context** out)
   g.nz = (int)p->n[2];
   g.plane = (uint64_t)g.nx * g.ny;
   g.nv = g.plane * g.nz;
+  g.ns = p->num_species;
   auto bail = [&](int code) { free_ctx(c); return code; };
   if (hipGetDevice(&c->device) != hipSuccess) return bail(set_err(CS_EDEVICE, "no HIP device"));
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
@@ -1688,7 +1764,7 @@ int cs_create(const cs_problem* p, cs_context** out)
   if (hipMalloc(&c->d_partial, sizeof(double) * STAT_BLOCKS * MAXS * MAXD * 3) != hipSuccess ||
       hipMalloc(&c->d_stats, sizeof(double) * MAXS * MAXD * 3) != hipSuccess)
     return bail(set_err(CS_ENOMEM, "device allocation of reduction buffers failed"));
-  c->launch = default_launch(g);
+  c->launch = default_launch(g, c->esize);
   rc = c->esize == 8 ? fill_init<double>(c) : fill_init<float>(c);
   if (rc != CS_OK) return bail(rc);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(set_err(CS_EDEVICE, "setup failed"));
@@ -1731,8 +1807,11 @@ int cs_upload_species(cs_context* c, uint32_t s, const double* src)
         if (c->esize == 8) ((double*)buf.data())[dv] = val;
         else ((float*)buf.data())[dv] = (float)val;
       }
-  char* base = (char*)c->d_u[c->cur] + (uint64_t)s * nv * c->esize;
-  HIP_TRY(hipMemcpyAsync(base, buf.data(), buf.size(), hipMemcpyHostToDevice, c->stream));
+  /* one plane of species s per z-group of the device stack */
+  const size_t pb = c->geo.plane * c->esize;
+  char* base = (char*)c->d_u[c->cur] + (uint64_t)s * pb;
+  HIP_TRY(hipMemcpy2DAsync(base, pb * c->ns, buf.data(), pb, pb, (size_t)c->geo.nz, hipMemcpyHostToDevice,
+                           c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return CS_OK;
 }
@@ -1744,8 +1823,10 @@ int cs_download_species(cs_context* c, uint32_t s, double* dst)
   HIP_TRY(hipSetDevice(c->device));
   const uint64_t nv = c->geo.nv;
   std::vector<char> buf(nv * c->esize);
-  const char* base = (const char*)c->d_u[c->cur] + (uint64_t)s * nv * c->esize;
-  HIP_TRY(hipMemcpyAsync(buf.data(), base, buf.size(), hipMemcpyDeviceToHost, c->stream));
+  const size_t pb = c->geo.plane * c->esize;
+  const char* base = (const char*)c->d_u[c->cur] + (uint64_t)s * pb;
+  HIP_TRY(hipMemcpy2DAsync(buf.data(), pb, base, pb * c->ns, pb, (size_t)c->geo.nz, hipMemcpyDeviceToHost,
+                           c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   const uint32_t n0 = c->n[0], n1 = c->n[1], n2 = c->n[2];
   for (uint32_t x = 0; x < n0; x++)
@@ -1806,15 +1887,18 @@ int cs_reduce_stats(cs_context* c, double* mn, double* mx, double* tot)
 {
   if (!c || !mn || !mx || !tot) return set_err(CS_EINVAL, "NULL argument");
   HIP_TRY(hipSetDevice(c->device));
-  const uint32_t blocks = (uint32_t)std::min<uint64_t>(STAT_BLOCKS, (c->geo.nv + STAT_THREADS - 1) / STAT_THREADS);
+  /* a 2-D grid of at most STAT_BLOCKS blocks: columns of a plane by planes */
+  const uint32_t gx = (uint32_t)std::min<uint64_t>(16, (c->geo.plane + STAT_THREADS - 1) / STAT_THREADS);
+  const uint32_t gy = (uint32_t)std::min<int>(STAT_BLOCKS / 16, c->geo.nz);
+  const uint32_t blocks = gx * gy;
   const double dV = c->prob.dV;
   if (c->esize == 8)
-    hipLaunchKernelGGL(k_stats<double>, dim3(blocks), dim3(STAT_THREADS), 0, c->stream,
+    hipLaunchKernelGGL(k_stats<double>, dim3(gx, gy), dim3(STAT_THREADS), 0, c->stream,
                        (const double*)c->d_u[c->cur], c->d_ids, c->geo, c->ns, dV, c->d_partial);
   else
-    hipLaunchKernelGGL(k_stats<float>, dim3(blocks), dim3(STAT_THREADS), 0, c->stream,
+    hipLaunchKernelGGL(k_stats<float>, dim3(gx, gy), dim3(STAT_THREADS), 0, c->stream,
                        (const float*)c->d_u[c->cur], c->d_ids, c->geo, c->ns, dV, c->d_partial);
-  hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(MAXS * MAXD), 0, c->stream, c->d_partial, blocks,
+  hipLaunchKernelGGL(k_stats_final, dim3(c->ns * MAXD), dim3(STAT_THREADS), 0, c->stream, c->d_partial, blocks,
                      c->ns, c->d_stats);
   HIP_TRY(hipGetLastError());
   double host[MAXS * MAXD * 3];
@@ -1847,16 +1931,16 @@ int cs_linescan(cs_context* c, uint32_t species, uint32_t axis, const double* wx
   if (!c->d_lpart) HIP_TRY(hipMalloc(&c->d_lpart, sizeof(double) * std::max(nrows, ncols)));
   double* dwx = c->d_lweights;
   double* dwy = dwx + g.nx;
-  const void* us = (const char*)c->d_u[c->cur] + (size_t)species * g.nv * c->esize;
+  const void* us = c->d_u[c->cur];
   std::vector<double> part;
   if (axis == 0) {
     HIP_TRY(hipMemcpyAsync(dwy, wy, sizeof(double) * g.ny, hipMemcpyHostToDevice, c->stream));
     if (c->esize == 8)
       hipLaunchKernelGGL(k_linescan_cols<double>, dim3(g.nz, gyc), dim3(256), 0, c->stream, (const double*)us,
-                         c->d_ids, g, domain_mask, dwy, ychunk, c->d_lpart);
+                         species, c->d_ids, g, domain_mask, dwy, ychunk, c->d_lpart);
     else
       hipLaunchKernelGGL(k_linescan_cols<float>, dim3(g.nz, gyc), dim3(256), 0, c->stream, (const float*)us,
-                         c->d_ids, g, domain_mask, dwy, ychunk, c->d_lpart);
+                         species, c->d_ids, g, domain_mask, dwy, ychunk, c->d_lpart);
     HIP_TRY(hipGetLastError());
     part.resize(ncols);
     HIP_TRY(hipMemcpyAsync(part.data(), c->d_lpart, sizeof(double) * ncols, hipMemcpyDeviceToHost, c->stream));
@@ -1875,10 +1959,10 @@ int cs_linescan(cs_context* c, uint32_t species, uint32_t axis, const double* wx
   HIP_TRY(hipMemcpyAsync(dwx, wx, sizeof(double) * g.nx, hipMemcpyHostToDevice, c->stream));
   const unsigned blocks = (unsigned)((nrows + 3) / 4);
   if (c->esize == 8)
-    hipLaunchKernelGGL(k_linescan_rows<double>, dim3(blocks), dim3(256), 0, c->stream, (const double*)us, c->d_ids,
+    hipLaunchKernelGGL(k_linescan_rows<double>, dim3(blocks), dim3(256), 0, c->stream, (const double*)us, species, c->d_ids,
                        g, domain_mask, dwx, c->d_lpart);
   else
-    hipLaunchKernelGGL(k_linescan_rows<float>, dim3(blocks), dim3(256), 0, c->stream, (const float*)us, c->d_ids,
+    hipLaunchKernelGGL(k_linescan_rows<float>, dim3(blocks), dim3(256), 0, c->stream, (const float*)us, species, c->d_ids,
                        g, domain_mask, dwx, c->d_lpart);
   HIP_TRY(hipGetLastError());
   part.resize(nrows);
@@ -1953,14 +2037,17 @@ int cs_set_launch_config(cs_context* c, const cs_launch_config* in)
   }
   if (in->zchunk < 1) return set_err(CS_EINVAL, "zchunk must be >= 1");
   if (in->prefetch != 1 && in->prefetch != 2) return set_err(CS_EINVAL, "prefetch must be 1 or 2");
-  if (in->xcd_remap > 1 || in->nontemporal > 1 || in->lds_tile > 1 || in->linear > 1)
+  if (in->nontemporal > 1 || in->lds_tile > 1 || in->linear > 1)
     return set_err(CS_EINVAL, "flags must be 0 or 1");
+  if (in->xcd_remap > 64) return set_err(CS_EINVAL, "xcd_remap is 0, 1 or a group size up to 64");
   if (in->lds_tile && in->linear) return set_err(CS_EINVAL, "the LDS tile kernel uses 2-D tiles");
   if (in->lds_tile && in->prefetch != 1) return set_err(CS_EINVAL, "the LDS tile kernel prefetches one plane");
   if (in->lds_tile && lds_tile_bytes(c->esize, c->ns, (int)in->rows) > LDS_TILE_LIMIT)
     return set_err(CS_EINVAL, "LDS tile of %u rows too large for %u species", in->rows, c->ns);
   LaunchCfg L;
   L.rows = (int)in->rows;
+  if (!in->lds_tile && (int)std::min<uint32_t>(in->zchunk, (uint32_t)c->geo.nz) > max_zchunk(c->geo, c->esize))
+    return set_err(CS_EINVAL, "zchunk %u exceeds %d for this grid", in->zchunk, max_zchunk(c->geo, c->esize));
   L.zchunk = (int)std::min<uint32_t>(in->zchunk, (uint32_t)c->geo.nz);
   L.xcd = (int)in->xcd_remap;
   L.pf = (int)in->prefetch;
@@ -2003,12 +2090,14 @@ int cs_autotune(cs_context* c, uint64_t first_time_ind, uint32_t reps, cs_launch
   }
   const int nz = c->geo.nz;
   std::vector<int> zcs;
-  for (int zc : {8, 16, 25, 32, 50, 64})
-    if (zc < nz) zcs.push_back(zc);
-  zcs.push_back(nz);
+  const int zmax = max_zchunk(c->geo, c->esize);
+  for (int zc : {8, 12, 16, 24, 32, 50, 64})
+    if (zc < nz && zc <= zmax) zcs.push_back(zc);
+  if (nz <= zmax) zcs.push_back(nz);
+  if (zcs.empty()) zcs.push_back(zmax);
   std::vector<LaunchCfg> cands;
   for (int zc : zcs)
-    for (int xcd : {0, 1})
+    for (int xcd : {0, 1, 4, 16})
       for (int nt : {0, 1}) {
         for (int pf : {1, 2})
           for (int rows : {2, 4, 8}) {

@@ -170,8 +170,10 @@ int cs_time_steps(cs_context* ctx, uint64_t first_time_ind, uint64_t nsteps,
                   double* total_ms, double* stencil_kernel_ms);
 
 /* Launch shape of the fused stencil/reaction kernel: blocks of 64 x rows
- * threads march zchunk planes; xcd_remap orders tiles so each of the 8
- * XCDs works on a contiguous slab (shared L2 for halo rows); prefetch is
+ * threads march zchunk planes; xcd_remap orders tiles over the 8 XCDs so
+ * that neighbouring tiles share an L2 for their halo rows (0: hardware
+ * order, 1: one contiguous slab per XCD, G >= 2: groups of G consecutive
+ * tiles per XCD, round-robin); prefetch is
  * the number of planes loaded ahead (1 or 2; 1 with lds_tile);
  * nontemporal streams the next
  * state past the caches; lds_tile stages each plane in LDS so the x/y

@@ -190,6 +190,9 @@ LAUNCHES = [{**c, "yblock": 0} for c in LAUNCHES] + [
     dict(rows=1, zchunk=3, xcd_remap=1, prefetch=1, nontemporal=1, lds_tile=0, linear=0, yblock=2),
     dict(rows=2, zchunk=64, xcd_remap=0, prefetch=2, nontemporal=0, lds_tile=0, linear=0, yblock=4),
     dict(rows=1, zchunk=1, xcd_remap=0, prefetch=2, nontemporal=1, lds_tile=0, linear=0, yblock=4),
+    dict(rows=2, zchunk=4, xcd_remap=3, prefetch=1, nontemporal=1, lds_tile=0, linear=1, yblock=0),
+    dict(rows=1, zchunk=2, xcd_remap=16, prefetch=2, nontemporal=0, lds_tile=0, linear=0, yblock=0),
+    dict(rows=1, zchunk=3, xcd_remap=5, prefetch=1, nontemporal=1, lds_tile=0, linear=0, yblock=2),
 ]
 
 
