@@ -248,6 +248,7 @@ def device_lib() -> C.CDLL:
         lib.cs_uniforms_needed.argtypes = [P, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64)]
         lib.cs_synchronize.argtypes = [P]
         lib.cs_reduce_stats.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        lib.cs_get_channel_recruited.argtypes = [P, C.c_uint32, C.POINTER(C.c_uint8)]
         lib.cs_linescan.argtypes = [P, C.c_uint32, C.c_uint32] + [C.POINTER(C.c_double)] * 3 + \
             [C.c_uint32, C.POINTER(C.c_double)]
         lib.cs_get_channel_states.argtypes = [P, C.c_uint32, C.POINTER(C.c_uint8)]
@@ -762,6 +763,15 @@ class Simulation:
                "cs_linescan")
         k = np.arange(n[axis]) * h
         return np.array([float(np.dot(proj, gauss(p - k, conv[axis]))) for p in k])
+
+    def recruited(self, flux: int) -> np.ndarray:
+        """Channels of discrete flux `flux` open at some gating step so far
+        (cs_get_channel_recruited)."""
+        n = self.setup.problem.fluxes[flux].num_pairs
+        out = np.zeros(max(n, 1), dtype=np.uint8)
+        _check(device_lib().cs_get_channel_recruited(self.ctx, flux, out.ctypes.data_as(C.POINTER(C.c_uint8))),
+               "cs_get_channel_recruited")
+        return out[:n]
 
     def channel_states(self, flux: int) -> np.ndarray:
         n = self.setup.problem.fluxes[flux].num_pairs

@@ -930,6 +930,7 @@ k_stencil_rows(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __res
 template <typename T>
 __global__ void __launch_bounds__(GATE_THREADS)
 k_gate(const T* __restrict__ u, Geo g, const uint32_t* __restrict__ pairs, uint8_t* __restrict__ states,
+       uint8_t* __restrict__ recruited,
        const DevParams<T>* __restrict__ P, uint64_t ti_arg, const uint64_t* __restrict__ ti_ptr,
        uint32_t D, uint32_t seed, uint32_t instance, const double* __restrict__ uniforms,
        uint64_t uniform_base)
@@ -956,6 +957,7 @@ k_gate(const T* __restrict__ u, Geo g, const uint32_t* __restrict__ pairs, uint8
         else if (st == 1) { if (r <= dts * kc) st = 0; }
       }
       states[F.state_off + ch] = st;
+      if (st) recruited[F.state_off + ch] = 1;
     }
   }
 }
@@ -1193,6 +1195,7 @@ struct cs_context {
   /* boundary fluxes */
   uint32_t* d_pairs = nullptr;
   uint8_t* d_states = nullptr;
+  uint8_t* d_recruited = nullptr;   /* channel open at any gating step so far */
   uint32_t nrows = 0;
   uint32_t* d_row_voxel = nullptr;
   uint8_t* d_row_species = nullptr;
@@ -1253,7 +1256,7 @@ void free_ctx(cs_context* c)
 {
   if (!c) return;
   hipSetDevice(c->device);
-  void* ptrs[] = {c->d_u[0], c->d_u[1], c->d_ids, c->d_params, c->d_pairs, c->d_states,
+  void* ptrs[] = {c->d_u[0], c->d_u[1], c->d_ids, c->d_params, c->d_pairs, c->d_states, c->d_recruited,
                   c->d_row_voxel, c->d_row_species, c->d_row_start, c->d_ent_pair,
                   c->d_ent_flux, c->d_ent_sign, c->d_uniforms, c->d_partial, c->d_stats,
                   c->d_lweights, c->d_lpart};
@@ -1414,9 +1417,12 @@ int fill_init(cs_context* c)
     hipLaunchKernelGGL(k_fill_init<T>, dim3(blocks), dim3(256), 0, c->stream, (T*)c->d_u[b],
                        c->d_ids, (const DevParams<T>*)c->d_params, c->geo, c->ns);
   HIP_TRY(hipGetLastError());
-  if (c->ndiscrete && !c->init_states.empty())
+  if (c->ndiscrete && !c->init_states.empty()) {
     HIP_TRY(hipMemcpyAsync(c->d_states, c->init_states.data(), c->init_states.size(),
                            hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_recruited, c->init_states.data(), c->init_states.size(),
+                           hipMemcpyHostToDevice, c->stream));
+  }
   c->cur = 0;
   return CS_OK;
 }
@@ -1524,7 +1530,7 @@ int step_one(cs_context* c, uint64_t ti, const double* d_uni, uint64_t uni_base,
   const T* ucur = (const T*)c->d_u[c->cur];
   if (c->ndiscrete && ti % p.stochastic_substep == 0) {
     hipLaunchKernelGGL(k_gate<T>, dim3(1), dim3(GATE_THREADS), 0, c->stream, ucur, c->geo, c->d_pairs,
-                       c->d_states, (const DevParams<T>*)c->d_params, ti, (const uint64_t*)nullptr,
+                       c->d_states, c->d_recruited, (const DevParams<T>*)c->d_params, ti, (const uint64_t*)nullptr,
                        p.stochastic_substep, (uint32_t)p.seed, (uint32_t)p.instance, d_uni, uni_base);
   }
   if (e0) HIP_TRY(hipEventRecord(e0, c->stream));
@@ -1681,6 +1687,7 @@ int build_flux_rows(cs_context* c, const cs_problem* p)
     if (!(F.params[8] > 0)) c->stoch_per_gate += F.num_pairs;
   }
   if ((rc = up(&c->d_states, c->init_states)) != CS_OK) return rc;
+  if ((rc = up(&c->d_recruited, c->init_states)) != CS_OK) return rc;
   return CS_OK;
 }
 
@@ -2000,10 +2007,28 @@ int cs_set_channel_states(cs_context* c, uint32_t f, const uint8_t* states)
   if (!c || !states) return set_err(CS_EINVAL, "NULL argument");
   if (f >= c->nf || !c->prob.fluxes[f].discrete) return set_err(CS_ERANGE, "flux %u is not a discrete flux", f);
   HIP_TRY(hipSetDevice(c->device));
-  std::vector<uint8_t> tmp(states, states + c->npairs[f]);
+  std::vector<uint8_t> tmp(states, states + c->npairs[f]), rec(c->npairs[f]);
   for (auto& v : tmp) v = v ? 1 : 0;
-  if (c->npairs[f])
+  if (c->npairs[f]) {
     HIP_TRY(hipMemcpyAsync(c->d_states + c->state_off[f], tmp.data(), c->npairs[f], hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(rec.data(), c->d_recruited + c->state_off[f], c->npairs[f], hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (uint32_t i = 0; i < c->npairs[f]; i++) rec[i] |= tmp[i];
+    HIP_TRY(hipMemcpyAsync(c->d_recruited + c->state_off[f], rec.data(), c->npairs[f], hipMemcpyHostToDevice,
+                           c->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return CS_OK;
+}
+
+int cs_get_channel_recruited(cs_context* c, uint32_t f, uint8_t* out)
+{
+  if (!c || !out) return set_err(CS_EINVAL, "NULL argument");
+  if (f >= c->nf || !c->prob.fluxes[f].discrete) return set_err(CS_ERANGE, "flux %u is not a discrete flux", f);
+  HIP_TRY(hipSetDevice(c->device));
+  if (c->npairs[f])
+    HIP_TRY(hipMemcpyAsync(out, c->d_recruited + c->state_off[f], c->npairs[f], hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return CS_OK;
 }

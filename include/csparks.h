@@ -163,6 +163,13 @@ int cs_linescan(cs_context* ctx, uint32_t species, uint32_t axis, const double* 
 int cs_get_channel_states(cs_context* ctx, uint32_t flux, uint8_t* states);
 int cs_set_channel_states(cs_context* ctx, uint32_t flux, const uint8_t* states);
 
+/* Channels of discrete flux f that were open at the start or after any
+ * gating step since cs_create / cs_reset (states set by
+ * cs_set_channel_states count too): the RyRs a spark recruited.  No
+ * reference counterpart; the reference reports only the current states
+ * (discrete_<name> datasets, species.c:3050-3080). */
+int cs_get_channel_recruited(cs_context* ctx, uint32_t flux, uint8_t* recruited);
+
 /* Time nsteps steps on the context stream with HIP events: total wall
  * time of the range and the mean duration of the fused stencil/reaction
  * kernel launches (ms).  Uses Philox gating. */

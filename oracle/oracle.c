@@ -23,6 +23,7 @@ typedef struct obnd {
   uint64_t* from;          /* refined linear indices                      */
   uint64_t* to;
   uint8_t* open;           /* discrete only                               */
+  uint8_t* ever;           /* open at any gating step so far (discrete)   */
   int      gate_group;     /* ordinal among gated discrete boundaries     */
   double   params[10];
   int      dsi;            /* diffusive index of the flux species         */
@@ -197,7 +198,9 @@ void oracle_free(oracle_sim* s)
   if (!s) return;
   free(s->ids);
   for (uint32_t i = 0; i < NS_MAX; i++) { free(s->u1[i]); free(s->du[i]); }
-  for (uint32_t b = 0; b < CS_MAX_BOUNDARIES; b++) { free(s->bnd[b].from); free(s->bnd[b].to); free(s->bnd[b].open); }
+  for (uint32_t b = 0; b < CS_MAX_BOUNDARIES; b++) {
+    free(s->bnd[b].from); free(s->bnd[b].to); free(s->bnd[b].open); free(s->bnd[b].ever);
+  }
   free(s);
 }
 
@@ -331,6 +334,7 @@ int oracle_create(const cs_geometry* g, const cs_species_model* m, const cs_opti
     if (B->discrete) {
       s->disc_b[s->ndisc++] = b;
       B->open = (uint8_t*)calloc(B->npairs + 1, 1);
+      B->ever = (uint8_t*)calloc(B->npairs + 1, 1);
       if (B->model >= 0) B->gate_group = gate++;
     }
     if (B->model == 0 || B->model == 1) {
@@ -389,6 +393,10 @@ int oracle_create(const cs_geometry* g, const cs_species_model* m, const cs_opti
         }
       }
     }
+  }
+  for (uint32_t k = 0; k < s->ndisc; k++) {
+    obnd* B = &s->bnd[s->disc_b[k]];
+    memcpy(B->ever, B->open, B->npairs);
   }
   *out = s;
   return 0;
@@ -475,6 +483,7 @@ int oracle_step(oracle_sim* s, uint64_t ti)
     for (uint32_t k = 0; k < s->ndisc; k++) {
       obnd* B = &s->bnd[s->disc_b[k]];
       if (B->model == 0 || B->model == 1) gate_boundary(s, B, ti);
+      for (uint32_t i = 0; i < B->npairs; i++) B->ever[i] |= B->open[i];
     }
   /* Species_step_boundary_fluxes (species.c:1704-1807) */
   for (uint32_t b = 0; b < s->nb_geom; b++) {
@@ -621,5 +630,17 @@ int oracle_set_states(oracle_sim* s, uint32_t k, const uint8_t* in)
   if (k >= s->ndisc) return -1;
   obnd* B = &s->bnd[s->disc_b[k]];
   memcpy(B->open, in, B->npairs);
+  for (uint32_t i = 0; i < B->npairs; i++) B->ever[i] |= B->open[i];
+  return 0;
+}
+
+/* Channels of discrete boundary k open at the start or after any gating
+ * step so far: the RyRs a spark recruited (a statistic of this project;
+ * the reference reports none). */
+int oracle_get_recruited(const oracle_sim* s, uint32_t k, uint8_t* out)
+{
+  if (k >= s->ndisc) return -1;
+  const obnd* B = &s->bnd[s->disc_b[k]];
+  memcpy(out, B->ever, B->npairs);
   return 0;
 }

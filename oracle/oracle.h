@@ -69,6 +69,8 @@ void oracle_linescan(const oracle_sim* s, uint32_t species, uint32_t axis, const
 /* Channel states of discrete boundary k (file order among discrete). */
 int  oracle_get_states(const oracle_sim* s, uint32_t k, uint8_t* out);
 int  oracle_set_states(oracle_sim* s, uint32_t k, const uint8_t* in);
+/* channels of discrete boundary k ever open since creation */
+int  oracle_get_recruited(const oracle_sim* s, uint32_t k, uint8_t* out);
 
 /* Reference scalar functions, restated (boundaryfluxes.c:12-141). */
 double oracle_ryr_flux(double dt, double h, double u0, double u1, const double* p);

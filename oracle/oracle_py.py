@@ -63,6 +63,7 @@ def lib() -> C.CDLL:
         L.oracle_linescan.restype = None
         L.oracle_get_states.argtypes = [P, C.c_uint32, C.POINTER(C.c_uint8)]
         L.oracle_set_states.argtypes = [P, C.c_uint32, C.POINTER(C.c_uint8)]
+        L.oracle_get_recruited.argtypes = [P, C.c_uint32, C.POINTER(C.c_uint8)]
         L.oracle_ryr_flux.argtypes = [C.c_double] * 4 + [C.POINTER(C.c_double)]
         L.oracle_ryr_flux.restype = C.c_double
         L.oracle_serca_flux.argtypes = [C.c_double] * 4 + [C.POINTER(C.c_double)]
@@ -141,6 +142,14 @@ class Oracle:
         n = int(self.info.discrete_size[k])
         out = np.zeros(max(n, 1), dtype=np.uint8)
         if lib().oracle_get_states(self.ptr, k, out.ctypes.data_as(C.POINTER(C.c_uint8))) != 0:
+            raise IndexError(k)
+        return out[:n]
+
+    def recruited(self, k: int) -> np.ndarray:
+        """Channels of discrete boundary k open at some gating step so far."""
+        n = int(self.info.discrete_size[k])
+        out = np.zeros(max(n, 1), dtype=np.uint8)
+        if lib().oracle_get_recruited(self.ptr, k, out.ctypes.data_as(C.POINTER(C.c_uint8))) != 0:
             raise IndexError(k)
         return out[:n]
 

@@ -1,0 +1,49 @@
+"""Spark statistics of GPU realisations against the oracle's.
+
+The oracle realisations (tests/golden/spark_stats_fast_dyad.json) gate
+with the reference's Mersenne Twister order in FP64; the GPU ones use the
+Philox stream with FP32 fields.  The random streams differ, so the
+realisations differ one by one; the distributions of peak dF/F0, duration
+and RyRs recruited must agree (two-sample Kolmogorov-Smirnov, p > 0.05)."""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+from scipy.stats import ks_2samp
+
+import csparks as cs
+import dyad
+import sparks
+from conftest import REPO
+
+pytestmark = pytest.mark.gpu
+FIXTURE = os.path.join(REPO, "tests", "golden", "spark_stats_fast_dyad.json")
+
+
+def test_spark_statistics_match_the_oracle_distributions():
+    with open(FIXTURE) as f:
+        fx = json.load(f)
+    ref = fx["realisations"]
+    d = dyad.fast_spark_dyad()
+    n = len(ref)
+    got = []
+    batch = 16
+    for b0 in range(1, n + 1, batch):
+        seeds = range(b0, min(n, b0 + batch - 1) + 1)
+        print(f"spark realisations seeds {seeds.start}..{seeds.stop - 1}", flush=True)
+        sims = [cs.Simulation(d.geometry, d.model,
+                              cs.Options(seed=s, precision="f32", open={"ryr": fx["open"]["ryr"]}))
+                for s in seeds]
+        got += sparks.realise_batch(sims, fx["nsteps"], fx["save_every"], fx["species"], fx["offsets"],
+                                    d.model.conv)
+        for s in sims:
+            s.close()
+    for key in ("peak", "duration", "recruited"):
+        a = np.array([getattr(g, key) for g in got])
+        b = np.array([r[key] for r in ref])
+        res = ks_2samp(a, b)
+        print(f"{key}: GPU mean {a.mean():.4g}, oracle mean {b.mean():.4g}, KS D {res.statistic:.3f} p {res.pvalue:.3f}",
+              flush=True)
+        assert res.pvalue > 0.05, f"{key}: KS p = {res.pvalue:.4f}"

@@ -1,0 +1,55 @@
+"""Spark statistics definitions (CPU only)."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import dyad
+import sparks
+from conftest import REPO
+
+FIXTURE = os.path.join(REPO, "tests", "golden", "spark_stats_fast_dyad.json")
+
+
+def test_fdhm_of_a_triangle():
+    t = np.linspace(0.0, 10.0, 1001)
+    y = np.maximum(0.0, 1.0 - np.abs(t - 4.0) / 2.0)      # half height at t = 3 and 5
+    assert sparks.fdhm(t, y) == pytest.approx(2.0, abs=1e-9)
+
+
+def test_fdhm_of_a_gaussian_interpolates_between_samples():
+    t = np.linspace(0.0, 10.0, 41)
+    sigma = 1.3
+    y = np.exp(-((t - 5.1) ** 2) / (2 * sigma ** 2))
+    # closed form 2 sqrt(2 ln 2) sigma; linear interpolation on a 0.25 grid
+    assert sparks.fdhm(t, y) == pytest.approx(2 * math.sqrt(2 * math.log(2)) * sigma, rel=2e-2)
+
+
+def test_fdhm_censors_at_the_end_and_handles_no_signal():
+    t = np.linspace(0.0, 1.0, 11)
+    assert sparks.fdhm(t, np.linspace(0.0, 1.0, 11)) == pytest.approx(0.5, abs=1e-12)
+    assert sparks.fdhm(t, np.zeros(11)) == 0.0
+
+
+def test_line_through_cleft():
+    d = dyad.fast_spark_dyad()
+    vox = d.geometry.voxels
+    off = sparks.line_through(vox, d.model.domain_names.index("cleft"))
+    n = vox.shape
+    y, z = n[1] // 2 + off[1], n[2] // 2 + off[2]
+    cleft = d.model.domain_names.index("cleft")
+    # the line crosses the cleft under the cluster, where the channels are
+    assert vox[n[0] // 2, y, z] == cleft
+    assert np.sum(vox[:, y, z] == cleft) >= n[0] // 2
+
+
+def test_fixture_holds_the_oracle_realisations():
+    with open(FIXTURE) as f:
+        d = json.load(f)
+    rows = d["realisations"]
+    assert len(rows) == 300 and d["rng"] == "mt"
+    rec = np.array([r["recruited"] for r in rows])
+    # bimodal by construction: the first channel alone, or the whole cluster
+    assert 0.2 < np.mean(rec == 1) < 0.5 and 0.5 < np.mean(rec == 16) < 0.8
