@@ -78,18 +78,26 @@ struct FluxParams {
   double   p[CS_NUM_RYR_PARAMS]; /* reference parameter vector           */
 };
 
-/* Rates of the buffer that binds a species in one domain (zero when none
- * does); padded to 4 values so one LDS read fetches them. */
+/* The buffer that binds a species s in one domain d (zero when none
+ * does), about the resting state: a = tot - B0 with B0 the resting value
+ * of s, c0 the resting value of species 0, j0 the rate at rest (zero for
+ * equilibrium initial values).  Padded to 8 values. */
 template <typename T>
 struct RateTab {
-  T kon, koff, tot, pad;
+  T kon, koff, a, c0, j0, pad[3];
 };
 
 template <typename T>
 struct DevParams {
   T coef[MAXS][MAXD * MAXD];
-  T kon[MAXS][MAXD], koff[MAXS][MAXD], tot[MAXS][MAXD];
-  T init[MAXD][MAXS];
+  /* Fields hold deviations from the resting state init[domain][species]:
+   * dinit[s][d*MAXD+dn] = init[dn][s] - init[d][s] enters the stencil
+   * across a domain interface; the buffer tables are taken about rest
+   * (slot b, domain d): a = tot - B0, f0 = resting free species, j0 = rate
+   * at rest */
+  T kon[MAXS][MAXD], koff[MAXS][MAXD], ra[MAXS][MAXD], rf0[MAXS][MAXD], rj0[MAXS][MAXD];
+  T dinit[MAXS][MAXD * MAXD];
+  double init64[MAXD][MAXS];
   T dt_react;
   uint8_t bsp0[MAXS][MAXD], bsp1[MAXS][MAXD];
   uint32_t nb;
@@ -195,27 +203,29 @@ inline int xcd_launch(int nblocks, int xcd)
 template <typename T, int NS>
 struct BlockTables {
   T coef[NS][MAXD * MAXD];
-  T kon[MAXS][MAXD], koff[MAXS][MAXD], tot[MAXS][MAXD];
-  T init[MAXD][NS];
+  T dinit[NS][MAXD * MAXD];
+  T kon[MAXS][MAXD], koff[MAXS][MAXD], ra[MAXS][MAXD], rf0[MAXS][MAXD], rj0[MAXS][MAXD];
   RateTab<T> rt[MAXD][NS];
   uint8_t b0[MAXS][MAXD], b1[MAXS][MAXD];
   uint32_t fixed[MAXD];
 
   __device__ void load(const DevParams<T>* __restrict__ P, int tid, int nthr)
   {
-    for (int i = tid; i < NS * MAXD * MAXD; i += nthr) coef[i / (MAXD * MAXD)][i % (MAXD * MAXD)] = P->coef[i / (MAXD * MAXD)][i % (MAXD * MAXD)];
+    for (int i = tid; i < NS * MAXD * MAXD; i += nthr) {
+      coef[i / (MAXD * MAXD)][i % (MAXD * MAXD)] = P->coef[i / (MAXD * MAXD)][i % (MAXD * MAXD)];
+      dinit[i / (MAXD * MAXD)][i % (MAXD * MAXD)] = P->dinit[i / (MAXD * MAXD)][i % (MAXD * MAXD)];
+    }
     for (int i = tid; i < MAXS * MAXD; i += nthr) {
       const int b = i / MAXD, d = i % MAXD;
       kon[b][d] = P->kon[b][d];
       koff[b][d] = P->koff[b][d];
-      tot[b][d] = P->tot[b][d];
+      ra[b][d] = P->ra[b][d];
+      rf0[b][d] = P->rf0[b][d];
+      rj0[b][d] = P->rj0[b][d];
       b0[b][d] = P->bsp0[b][d];
       b1[b][d] = P->bsp1[b][d];
     }
-    for (int i = tid; i < MAXD * NS; i += nthr) {
-      init[i / NS][i % NS] = P->init[i / NS][i % NS];
-      rt[i / NS][i % NS] = P->rt[i / NS][i % NS];
-    }
+    for (int i = tid; i < MAXD * NS; i += nthr) rt[i / NS][i % NS] = P->rt[i / NS][i % NS];
     if (tid < MAXD) fixed[tid] = P->fixed_mask[tid];
   }
 };
@@ -238,7 +248,8 @@ __device__ inline void react_general(T nw[NS], const T uc[NS], const BlockTables
       if (s == b0) B0 = uc[s];
       if (s == b1) B1 = uc[s];
     }
-    const T J = S.kon[b][d] * (S.tot[b][d] - B0) * B1 - S.koff[b][d] * B0;
+    /* kon (tot - B) F - koff B about rest, B and F deviations */
+    const T J = S.rj0[b][d] + S.kon[b][d] * (S.ra[b][d] * B1 - B0 * (S.rf0[b][d] + B1)) - S.koff[b][d] * B0;
     const T inc = dtr * J;
 #pragma unroll
     for (int s = 0; s < NS; s++)
@@ -266,7 +277,7 @@ __device__ inline void react_species_major(T nw[NS], const T uc[NS], const Block
   for (int s = 1; s < NS; s++) {
     if (!((rbound >> s) & 1u)) continue;
     const RateTab<T> r = S.rt[d][s];
-    const T J = r.kon * (r.tot - uc[s]) * uc[0] - r.koff * uc[s];
+    const T J = r.j0 + r.kon * (r.a * uc[0] - uc[s] * (r.c0 + uc[0])) - r.koff * uc[s];
     const T inc = dtr * J;
     nw[s] += inc;
     f -= inc;
@@ -511,21 +522,25 @@ k_stencil(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict
           const T eym = nym[s] - c, eyp = nyp[s] - c;
           const T ezm = um[s] - c, ezp = up[s] - c;
           const T* C = S.coef[s] + d * MAXD;
-          if (uni)
+          if (uni) {
             nw[s] = c + C[d] * ((ezm + ezp) + (eym + eyp) + (exm + exp_));
-          else
-            nw[s] = c + ((C[dzm] * ezm + C[dzp] * ezp) + (C[dym] * eym + C[dyp] * eyp) +
-                         (C[dxm] * exm + C[dxp] * exp_));
+          } else {
+            /* deviations of different domains differ by their resting values */
+            const T* D = S.dinit[s] + d * MAXD;
+            nw[s] = c + ((C[dzm] * (ezm + D[dzm]) + C[dzp] * (ezp + D[dzp])) +
+                         (C[dym] * (eym + D[dym]) + C[dyp] * (eyp + D[dyp])) +
+                         (C[dxm] * (exm + D[dxm]) + C[dxp] * (exp_ + D[dxp])));
+          }
         }
     }
     if (react == 2) react_species_major<T, NS>(nw, uc, S, d, rbound, dtr);
     else react_general<T, NS>(nw, uc, S, d, nb, dmask, dtr);
-    if (dirichlet && (xyface || z == 0 || !ztop)) nw[0] = S.init[d][0];
+    if (dirichlet && (xyface || z == 0 || !ztop)) nw[0] = T(0);   /* the resting value */
     const uint32_t fm = S.fixed[d];
     if (fm) {
 #pragma unroll
       for (int s = 0; s < NS; s++)
-        if ((fm >> s) & 1u) nw[s] = S.init[d][s];
+        if ((fm >> s) & 1u) nw[s] = T(0);
     }
     const uint32_t so = (uint32_t)(z - z0) * gbytes;
 #pragma unroll
@@ -691,20 +706,23 @@ k_stencil_lds(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __rest
           if (uni) {
             acc = c + C[d] * ((ezm + ezp) + (eym + eyp) + (exm + exp_));
           } else {
-            acc = c + ((C[dzm] * ezm + C[dzp] * ezp) + (C[dym] * eym + C[dyp] * eyp) +
-                       (C[dxm] * exm + C[dxp] * exp_));
+            /* deviations of different domains differ by their resting values */
+            const T* D = S.dinit[s] + d * MAXD;
+            acc = c + ((C[dzm] * (ezm + D[dzm]) + C[dzp] * (ezp + D[dzp])) +
+                       (C[dym] * (eym + D[dym]) + C[dyp] * (eyp + D[dyp])) +
+                       (C[dxm] * (exm + D[dxm]) + C[dxp] * (exp_ + D[dxp])));
           }
         }
         nw[s] = acc;
       }
       if (react == 2) react_species_major<T, NS>(nw, uc, S, d, rbound, dtr);
       else react_general<T, NS>(nw, uc, S, d, nb, dmask, dtr);
-      if (dirichlet && (xyface || z == 0 || !ztop)) nw[0] = S.init[d][0];
+      if (dirichlet && (xyface || z == 0 || !ztop)) nw[0] = T(0);   /* the resting value */
       const uint32_t fm = S.fixed[d];
       if (fm) {
 #pragma unroll
         for (int s = 0; s < NS; s++)
-          if ((fm >> s) & 1u) nw[s] = S.init[d][s];
+          if ((fm >> s) & 1u) nw[s] = T(0);
       }
 #pragma unroll
       for (int s = 0; s < NS; s++) store_next(un + ((uint64_t)z * NS + s) * g.plane + col, nw[s], NT);
@@ -893,20 +911,23 @@ k_stencil_rows(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __res
           if (uni) {
             acc = c + C[d] * ((ezm + ezp) + (eym + eyp) + (exm + exp_));
           } else {
-            acc = c + ((C[dzm] * ezm + C[dzp] * ezp) + (C[dym] * eym + C[dyp] * eyp) +
-                       (C[dxm] * exm + C[dxp] * exp_));
+            /* deviations of different domains differ by their resting values */
+            const T* D = S.dinit[s] + d * MAXD;
+            acc = c + ((C[dzm] * (ezm + D[dzm]) + C[dzp] * (ezp + D[dzp])) +
+                       (C[dym] * (eym + D[dym]) + C[dyp] * (eyp + D[dyp])) +
+                       (C[dxm] * (exm + D[dxm]) + C[dxp] * (exp_ + D[dxp])));
           }
         }
         nw[s] = acc;
       }
       if (react == 2) react_species_major<T, NS>(nw, uc, S, d, rbound, dtr);
       else react_general<T, NS>(nw, uc, S, d, nb, dmask, dtr);
-      if (dirichlet && (xface || yr == 0 || yr + 1 == g.ny || z == 0 || z == ztop_plane)) nw[0] = S.init[d][0];
+      if (dirichlet && (xface || yr == 0 || yr + 1 == g.ny || z == 0 || z == ztop_plane)) nw[0] = T(0);   /* the resting value */
       const uint32_t fm = S.fixed[d];
       if (fm) {
 #pragma unroll
         for (int s = 0; s < NS; s++)
-          if ((fm >> s) & 1u) nw[s] = S.init[d][s];
+          if ((fm >> s) & 1u) nw[s] = T(0);
       }
       if (xin && yr < g.ny) {
 #pragma unroll
@@ -929,8 +950,8 @@ k_stencil_rows(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __res
  * voxel (species.c:2569). */
 template <typename T>
 __global__ void __launch_bounds__(GATE_THREADS)
-k_gate(const T* __restrict__ u, Geo g, const uint32_t* __restrict__ pairs, uint8_t* __restrict__ states,
-       uint8_t* __restrict__ recruited,
+k_gate(const T* __restrict__ u, const uint8_t* __restrict__ ids, Geo g, const uint32_t* __restrict__ pairs,
+       uint8_t* __restrict__ states, uint8_t* __restrict__ recruited,
        const DevParams<T>* __restrict__ P, uint64_t ti_arg, const uint64_t* __restrict__ ti_ptr,
        uint32_t D, uint32_t seed, uint32_t instance, const double* __restrict__ uniforms,
        uint64_t uniform_base)
@@ -948,7 +969,7 @@ k_gate(const T* __restrict__ u, Geo g, const uint32_t* __restrict__ pairs, uint8
         if (p[8] <= t + dts) st = 0;
       } else {
         const uint32_t to = pairs[2 * (F.pair_off + ch) + 1];
-        const double c0 = (double)u[elem(g, 0, to)];
+        const double c0 = (double)u[elem(g, 0, to)] + P->init64[ids[to] & 7][0];
         const double kc = cs_rate_close(c0, p);
         const double ko = cs_rate_open(c0, p);
         const double r = uniforms ? uniforms[uniform_base + F.uniform_off + ch]
@@ -994,12 +1015,14 @@ k_flux(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__ 
     const uint32_t p = ent_pair[e];
     if (F.discrete && !states[F.state_off + p]) continue;
     const uint32_t from = pairs[2 * (F.pair_off + p)], to = pairs[2 * (F.pair_off + p) + 1];
-    const T u0 = u[elem(g, s, from)], u1 = u[elem(g, s, to)];
+    /* absolute values: stored deviation + the voxel's resting value */
+    const double r0 = P->init64[ids[from] & 7][s], r1 = P->init64[ids[to] & 7][s];
+    const double u0 = (double)u[elem(g, s, from)] + r0, u1 = (double)u[elem(g, s, to)] + r1;
     double J;
     if (F.kind == CS_FLUX_RYR)
-      J = (double)((u0 - u1) * (T)F.half_1mK0);
+      J = (((double)u[elem(g, s, from)] - (double)u[elem(g, s, to)]) + (r0 - r1)) * F.half_1mK0;
     else
-      J = cs_serca_flux_d(dt, h, (double)u0, (double)u1, F.density);
+      J = cs_serca_flux_d(dt, h, u0, u1, F.density);
     sum += ent_sign[e] > 0 ? J : -J;
     any = true;
   }
@@ -1034,8 +1057,8 @@ __device__ inline void block_reduce3(double (&mn)[MAXD], double (&mx)[MAXD], dou
 
 template <typename T>
 __global__ void __launch_bounds__(STAT_THREADS)
-k_stats(const T* __restrict__ u, const uint8_t* __restrict__ ids, Geo g, uint32_t ns, double dV,
-        double* __restrict__ partial /* [grid][ns][MAXD][3] */)
+k_stats(const T* __restrict__ u, const uint8_t* __restrict__ ids, const DevParams<T>* __restrict__ P, Geo g,
+        uint32_t ns, double dV, double* __restrict__ partial /* [grid][ns][MAXD][3] */)
 {
   /* block (bx, by) takes planes by, by + gridDim.y, ... and within each the
    * columns bx*blockDim + t strided by the grid width: a fixed assignment,
@@ -1053,7 +1076,7 @@ k_stats(const T* __restrict__ u, const uint8_t* __restrict__ ids, Geo g, uint32_
       for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < g.plane;
            c += (uint64_t)gridDim.x * blockDim.x) {
         const int dv = iz[c] & 7;
-        const double val = (double)us[c];
+        const double val = (double)us[c] + P->init64[dv][s];   /* deviation + resting value */
 #pragma unroll
         for (int d = 0; d < MAXD; d++)
           if (d == dv) {
@@ -1118,8 +1141,9 @@ k_stats_final(const double* __restrict__ partial, uint32_t nblocks, uint32_t ns,
  * wavefront per (y, z) row, butterfly reduction across the wavefront. */
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_linescan_rows(const T* __restrict__ u, uint32_t s, const uint8_t* __restrict__ ids, Geo g, uint32_t dmask,
-                const double* __restrict__ wx, double* __restrict__ rows)
+k_linescan_rows(const T* __restrict__ u, uint32_t s, const uint8_t* __restrict__ ids,
+                const DevParams<T>* __restrict__ P, Geo g, uint32_t dmask, const double* __restrict__ wx,
+                double* __restrict__ rows)
 {
   const uint64_t row = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
   if (row >= (uint64_t)g.ny * (uint64_t)g.nz) return;   /* whole wavefront */
@@ -1128,8 +1152,10 @@ k_linescan_rows(const T* __restrict__ u, uint32_t s, const uint8_t* __restrict__
   const uint64_t ib = row * (uint64_t)g.nx;                                 /* ids   */
   const uint64_t ub = ((z * g.ns + s) * g.ny + y) * (uint64_t)g.nx;       /* field */
   double acc = 0.0;
-  for (int x = lane; x < g.nx; x += 64)
-    if ((dmask >> (ids[ib + x] & 7)) & 1u) acc += (double)u[ub + x] * wx[x];
+  for (int x = lane; x < g.nx; x += 64) {
+    const int d = ids[ib + x] & 7;
+    if ((dmask >> d) & 1u) acc += ((double)u[ub + x] + P->init64[d][s]) * wx[x];
+  }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
   if (lane == 0) rows[row] = acc;
 }
@@ -1139,8 +1165,9 @@ k_linescan_rows(const T* __restrict__ u, uint32_t s, const uint8_t* __restrict__
  * lane per x. */
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_linescan_cols(const T* __restrict__ u, uint32_t s, const uint8_t* __restrict__ ids, Geo g, uint32_t dmask,
-                const double* __restrict__ wy, int ychunk, double* __restrict__ part)
+k_linescan_cols(const T* __restrict__ u, uint32_t s, const uint8_t* __restrict__ ids,
+                const DevParams<T>* __restrict__ P, Geo g, uint32_t dmask, const double* __restrict__ wy, int ychunk,
+                double* __restrict__ part)
 {
   const int z = blockIdx.x, c = blockIdx.y;
   const int y0 = c * ychunk, y1 = min(g.ny, y0 + ychunk);
@@ -1149,7 +1176,8 @@ k_linescan_cols(const T* __restrict__ u, uint32_t s, const uint8_t* __restrict__
     double acc = 0.0;
     for (int y = y0; y < y1; y++) {
       const uint64_t v = ((uint64_t)z * g.ny + y) * g.nx + x;
-      if ((dmask >> (ids[v] & 7)) & 1u) acc += (double)us[(uint64_t)y * g.nx + x] * wy[y];
+      const int d = ids[v] & 7;
+      if ((dmask >> d) & 1u) acc += ((double)us[(uint64_t)y * g.nx + x] + P->init64[d][s]) * wy[y];
     }
     part[((uint64_t)z * gridDim.y + c) * g.nx + x] = acc;
   }
@@ -1164,16 +1192,15 @@ __global__ void k_calibration_copy(const uint4* __restrict__ a, uint4* __restric
     b[i] = a[i];
 }
 
-/* Fill every voxel with init[domain][species]. */
+/* Fill every voxel with init[domain][species], i.e. a zero deviation from
+ * the resting state (zero fill kernel rather than a memset, so the fill can
+ * sit in a captured graph). */
 template <typename T>
-__global__ void k_fill_init(T* __restrict__ u, const uint8_t* __restrict__ ids,
-                            const DevParams<T>* __restrict__ P, Geo g, uint32_t ns)
+__global__ void k_fill_init(T* __restrict__ u, Geo g)
 {
-  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.nv;
-       v += (uint64_t)gridDim.x * blockDim.x) {
-    const int d = ids[v] & 7;
-    for (uint32_t s = 0; s < ns; s++) u[elem(g, s, v)] = P->init[d][s];
-  }
+  const uint64_t n = g.nv * g.ns;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    u[i] = T(0);
 }
 
 }  // namespace
@@ -1220,6 +1247,7 @@ struct cs_context {
   uint32_t flux_substep[MAXF] = {0};
   LaunchCfg launch{4, 32, 1, 1, 0, 0, 0, 0};
   uint32_t rfast = 0;   /* buffer reactions admit the species-major form */
+  std::vector<uint8_t> h_ids;   /* domain ids in device order (host copy)  */
 };
 
 namespace {
@@ -1278,18 +1306,28 @@ void fill_params(DevParams<T>& dp, const cs_problem* p, const cs_context* c)
   memset(&dp, 0, sizeof dp);
   for (int s = 0; s < MAXS; s++)
     for (int i = 0; i < MAXD * MAXD; i++) dp.coef[s][i] = (T)p->coef[s][i];
-  for (int b = 0; b < MAXS; b++)
-    for (int d = 0; d < MAXD; d++) {
-      dp.kon[b][d] = (T)p->k_on[b][d];
-      dp.koff[b][d] = (T)p->k_off[b][d];
-      dp.tot[b][d] = (T)p->tot[b][d];
-      dp.bsp0[b][d] = p->bsp0[b][d];
-      dp.bsp1[b][d] = p->bsp1[b][d];
-    }
+  /* resting state: the device stores u - init[domain][species]; the
+   * expansions about it are formed in double and rounded once */
   for (int d = 0; d < MAXD; d++)
     for (int s = 0; s < MAXS; s++) {
-      dp.init[d][s] = (T)p->init[d][s];
+      dp.init64[d][s] = p->init[d][s];
       if (p->fixed[d][s]) dp.fixed_mask[d] |= 1u << s;
+    }
+  for (int s = 0; s < MAXS; s++)
+    for (int d = 0; d < MAXD; d++)
+      for (int dn = 0; dn < MAXD; dn++) dp.dinit[s][d * MAXD + dn] = (T)(p->init[dn][s] - p->init[d][s]);
+  for (int b = 0; b < MAXS; b++)
+    for (int d = 0; d < MAXD; d++) {
+      const double kon = p->k_on[b][d], koff = p->k_off[b][d];
+      const double B0 = p->init[d][p->bsp0[b][d] % MAXS], F0 = p->init[d][p->bsp1[b][d] % MAXS];
+      const double a = p->tot[b][d] - B0;
+      dp.kon[b][d] = (T)kon;
+      dp.koff[b][d] = (T)koff;
+      dp.ra[b][d] = (T)a;
+      dp.rf0[b][d] = (T)F0;
+      dp.rj0[b][d] = (T)(kon * a * F0 - koff * B0);
+      dp.bsp0[b][d] = p->bsp0[b][d];
+      dp.bsp1[b][d] = p->bsp1[b][d];
     }
   /* species-major table: usable when every buffer slot in use binds
    * species 0 (its free species) and no domain binds a species twice */
@@ -1306,9 +1344,13 @@ void fill_params(DevParams<T>& dp, const cs_problem* p, const cs_context* c)
         continue;
       }
       seen |= 1u << b0;
-      dp.rt[d][b0].kon = (T)p->k_on[b][d];
-      dp.rt[d][b0].koff = (T)p->k_off[b][d];
-      dp.rt[d][b0].tot = (T)p->tot[b][d];
+      const double kon = p->k_on[b][d], koff = p->k_off[b][d];
+      const double B0 = p->init[d][b0], C0 = p->init[d][0], a = p->tot[b][d] - B0;
+      dp.rt[d][b0].kon = (T)kon;
+      dp.rt[d][b0].koff = (T)koff;
+      dp.rt[d][b0].a = (T)a;
+      dp.rt[d][b0].c0 = (T)C0;
+      dp.rt[d][b0].j0 = (T)(kon * a * C0 - koff * B0);
       dp.rbound |= 1u << b0;
     }
   }
@@ -1395,6 +1437,10 @@ int validate(const cs_problem* p)
                      (unsigned long long)v);
   for (uint32_t s = 0; s < p->num_species; s++)
     if (p->substeps[s] > 255) return set_err(CS_EINVAL, "substep too large");
+  for (uint32_t b = 0; b < p->num_buffers; b++)
+    for (uint32_t d = 0; d < p->num_domains; d++)
+      if (p->bsp0[b][d] >= p->num_species || p->bsp1[b][d] >= p->num_species)
+        return set_err(CS_ERANGE, "buffer %u of domain %u names a species out of range", b, d);
   return CS_OK;
 }
 
@@ -1414,8 +1460,7 @@ int fill_init(cs_context* c)
 {
   const int blocks = (int)std::min<uint64_t>(4096, (c->geo.nv + 255) / 256);
   for (int b = 0; b < 2; b++)
-    hipLaunchKernelGGL(k_fill_init<T>, dim3(blocks), dim3(256), 0, c->stream, (T*)c->d_u[b],
-                       c->d_ids, (const DevParams<T>*)c->d_params, c->geo, c->ns);
+    hipLaunchKernelGGL(k_fill_init<T>, dim3(blocks), dim3(256), 0, c->stream, (T*)c->d_u[b], c->geo);
   HIP_TRY(hipGetLastError());
   if (c->ndiscrete && !c->init_states.empty()) {
     HIP_TRY(hipMemcpyAsync(c->d_states, c->init_states.data(), c->init_states.size(),
@@ -1529,7 +1574,7 @@ int step_one(cs_context* c, uint64_t ti, const double* d_uni, uint64_t uni_base,
     if (ti % c->flux_substep[f] == 0) fluxmask |= 1u << f;
   const T* ucur = (const T*)c->d_u[c->cur];
   if (c->ndiscrete && ti % p.stochastic_substep == 0) {
-    hipLaunchKernelGGL(k_gate<T>, dim3(1), dim3(GATE_THREADS), 0, c->stream, ucur, c->geo, c->d_pairs,
+    hipLaunchKernelGGL(k_gate<T>, dim3(1), dim3(GATE_THREADS), 0, c->stream, ucur, c->d_ids, c->geo, c->d_pairs,
                        c->d_states, c->d_recruited, (const DevParams<T>*)c->d_params, ti, (const uint64_t*)nullptr,
                        p.stochastic_substep, (uint32_t)p.seed, (uint32_t)p.instance, d_uni, uni_base);
   }
@@ -1762,8 +1807,10 @@ int cs_create(const cs_problem* p, cs_context** out)
   for (int b = 0; b < 2; b++)
     if (hipMalloc(&c->d_u[b], c->esize * c->ns * g.nv) != hipSuccess)
       return bail(set_err(CS_ENOMEM, "device allocation of %zu bytes failed", c->esize * c->ns * g.nv));
-  if (hipMemcpyAsync(c->d_ids, ids.data(), g.nv, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+  if (hipMemcpyAsync(c->d_ids, ids.data(), g.nv, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
     return bail(set_err(CS_EDEVICE, "upload of ids failed"));
+  c->h_ids.swap(ids);
   rc = build_flux_rows(c, p);
   if (rc != CS_OK) return bail(rc);
   rc = c->esize == 8 ? upload_params<double>(c) : upload_params<float>(c);
@@ -1809,8 +1856,9 @@ int cs_upload_species(cs_context* c, uint32_t s, const double* src)
   for (uint32_t x = 0; x < n0; x++)
     for (uint32_t y = 0; y < n1; y++)
       for (uint32_t z = 0; z < n2; z++) {
-        const double val = src[((uint64_t)x * n1 + y) * n2 + z];
         const uint64_t dv = ((uint64_t)z * n1 + y) * n0 + x;
+        /* the device holds the deviation from the voxel's resting value */
+        const double val = src[((uint64_t)x * n1 + y) * n2 + z] - c->prob.init[c->h_ids[dv] & 7][s];
         if (c->esize == 8) ((double*)buf.data())[dv] = val;
         else ((float*)buf.data())[dv] = (float)val;
       }
@@ -1841,7 +1889,8 @@ int cs_download_species(cs_context* c, uint32_t s, double* dst)
       for (uint32_t z = 0; z < n2; z++) {
         const uint64_t dv = ((uint64_t)z * n1 + y) * n0 + x;
         dst[((uint64_t)x * n1 + y) * n2 + z] =
-            c->esize == 8 ? ((const double*)buf.data())[dv] : (double)((const float*)buf.data())[dv];
+            (c->esize == 8 ? ((const double*)buf.data())[dv] : (double)((const float*)buf.data())[dv]) +
+            c->prob.init[c->h_ids[dv] & 7][s];
       }
   return CS_OK;
 }
@@ -1901,10 +1950,12 @@ int cs_reduce_stats(cs_context* c, double* mn, double* mx, double* tot)
   const double dV = c->prob.dV;
   if (c->esize == 8)
     hipLaunchKernelGGL(k_stats<double>, dim3(gx, gy), dim3(STAT_THREADS), 0, c->stream,
-                       (const double*)c->d_u[c->cur], c->d_ids, c->geo, c->ns, dV, c->d_partial);
+                       (const double*)c->d_u[c->cur], c->d_ids, (const DevParams<double>*)c->d_params, c->geo, c->ns,
+                       dV, c->d_partial);
   else
     hipLaunchKernelGGL(k_stats<float>, dim3(gx, gy), dim3(STAT_THREADS), 0, c->stream,
-                       (const float*)c->d_u[c->cur], c->d_ids, c->geo, c->ns, dV, c->d_partial);
+                       (const float*)c->d_u[c->cur], c->d_ids, (const DevParams<float>*)c->d_params, c->geo, c->ns,
+                       dV, c->d_partial);
   hipLaunchKernelGGL(k_stats_final, dim3(c->ns * MAXD), dim3(STAT_THREADS), 0, c->stream, c->d_partial, blocks,
                      c->ns, c->d_stats);
   HIP_TRY(hipGetLastError());
@@ -1944,10 +1995,12 @@ int cs_linescan(cs_context* c, uint32_t species, uint32_t axis, const double* wx
     HIP_TRY(hipMemcpyAsync(dwy, wy, sizeof(double) * g.ny, hipMemcpyHostToDevice, c->stream));
     if (c->esize == 8)
       hipLaunchKernelGGL(k_linescan_cols<double>, dim3(g.nz, gyc), dim3(256), 0, c->stream, (const double*)us,
-                         species, c->d_ids, g, domain_mask, dwy, ychunk, c->d_lpart);
+                         species, c->d_ids, (const DevParams<double>*)c->d_params, g, domain_mask, dwy, ychunk,
+                         c->d_lpart);
     else
       hipLaunchKernelGGL(k_linescan_cols<float>, dim3(g.nz, gyc), dim3(256), 0, c->stream, (const float*)us,
-                         species, c->d_ids, g, domain_mask, dwy, ychunk, c->d_lpart);
+                         species, c->d_ids, (const DevParams<float>*)c->d_params, g, domain_mask, dwy, ychunk,
+                         c->d_lpart);
     HIP_TRY(hipGetLastError());
     part.resize(ncols);
     HIP_TRY(hipMemcpyAsync(part.data(), c->d_lpart, sizeof(double) * ncols, hipMemcpyDeviceToHost, c->stream));
@@ -1967,10 +2020,10 @@ int cs_linescan(cs_context* c, uint32_t species, uint32_t axis, const double* wx
   const unsigned blocks = (unsigned)((nrows + 3) / 4);
   if (c->esize == 8)
     hipLaunchKernelGGL(k_linescan_rows<double>, dim3(blocks), dim3(256), 0, c->stream, (const double*)us, species, c->d_ids,
-                       g, domain_mask, dwx, c->d_lpart);
+                       (const DevParams<double>*)c->d_params, g, domain_mask, dwx, c->d_lpart);
   else
     hipLaunchKernelGGL(k_linescan_rows<float>, dim3(blocks), dim3(256), 0, c->stream, (const float*)us, species, c->d_ids,
-                       g, domain_mask, dwx, c->d_lpart);
+                       (const DevParams<float>*)c->d_params, g, domain_mask, dwx, c->d_lpart);
   HIP_TRY(hipGetLastError());
   part.resize(nrows);
   HIP_TRY(hipMemcpyAsync(part.data(), c->d_lpart, sizeof(double) * nrows, hipMemcpyDeviceToHost, c->stream));
