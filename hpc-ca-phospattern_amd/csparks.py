@@ -173,7 +173,8 @@ class CSetup(C.Structure):
 class CLaunchConfig(C.Structure):
     _fields_ = [("rows", C.c_uint32), ("zchunk", C.c_uint32), ("xcd_remap", C.c_uint32),
                 ("prefetch", C.c_uint32), ("nontemporal", C.c_uint32), ("lds_tile", C.c_uint32),
-                ("linear", C.c_uint32), ("yblock", C.c_uint32)]
+                ("linear", C.c_uint32), ("yblock", C.c_uint32), ("fuse", C.c_uint32),
+                ("fuse_zchunk", C.c_uint32), ("fuse_xcd", C.c_uint32)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
@@ -254,6 +255,9 @@ def device_lib() -> C.CDLL:
         lib.cs_get_channel_states.argtypes = [P, C.c_uint32, C.POINTER(C.c_uint8)]
         lib.cs_set_channel_states.argtypes = [P, C.c_uint32, C.POINTER(C.c_uint8)]
         lib.cs_time_steps.argtypes = [P, C.c_uint64, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        lib.cs_time_passes.argtypes = [P, C.c_uint64, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        lib.cs_fused_available.argtypes = [P, C.c_char_p, C.c_size_t]
         lib.cs_device_info.argtypes = [C.POINTER(C.c_int), C.c_char_p, C.c_size_t]
         lib.cs_ryr_flux.argtypes = [C.c_double] * 4 + [C.POINTER(C.c_double)]
         lib.cs_ryr_flux.restype = C.c_double
@@ -803,12 +807,28 @@ class Simulation:
         return cfg.as_dict(), ms.value
 
     def time_steps(self, nsteps: int):
-        """Time nsteps steps with HIP events; returns (total_ms, mean stencil kernel ms)."""
+        """Time nsteps steps with HIP events; returns (total_ms, mean stencil pass ms)."""
         tot, ker = C.c_double(), C.c_double()
         _check(device_lib().cs_time_steps(self.ctx, self.time_ind, nsteps, C.byref(tot), C.byref(ker)),
                "cs_time_steps")
         self.time_ind += nsteps
         return tot.value, ker.value
+
+    def time_passes(self, nsteps: int):
+        """Time nsteps steps; returns (total_ms, mean pass ms, passes, fused passes)."""
+        tot, ker, npass, nfused = C.c_double(), C.c_double(), C.c_uint64(), C.c_uint64()
+        _check(device_lib().cs_time_passes(self.ctx, self.time_ind, nsteps, C.byref(tot), C.byref(ker),
+                                           C.byref(npass), C.byref(nfused)), "cs_time_passes")
+        self.time_ind += nsteps
+        return tot.value, ker.value, npass.value, nfused.value
+
+    def fused_available(self):
+        """(True, '') when the problem admits fused two-step passes, else (False, reason)."""
+        why = C.create_string_buffer(256)
+        ok = device_lib().cs_fused_available(self.ctx, why, 256)
+        if ok < 0:
+            _check(ok, "cs_fused_available")
+        return ok == 1, why.value.decode()
 
     def close(self):
         if getattr(self, "ctx", None) and self.ctx.value:

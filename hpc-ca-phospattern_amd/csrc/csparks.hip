@@ -33,6 +33,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "csparks.h"
@@ -46,6 +47,8 @@ constexpr int MAXF = CS_MAX_FLUXES;
 constexpr int BX = 64;          /* one wavefront along x                  */
 constexpr int GATE_THREADS = 256;
 constexpr int FLUX_THREADS = 256;
+constexpr int BRIDGE_THREADS = 1024;
+constexpr size_t F2_MAX_ROWS = 16384;   /* rows and channels k_bridge's one block takes */
 constexpr int STAT_THREADS = 256;
 constexpr int STAT_BLOCKS = 1024;
 
@@ -79,12 +82,13 @@ struct FluxParams {
 };
 
 /* The buffer that binds a species s in one domain d (zero when none
- * does), about the resting state: a = tot - B0 with B0 the resting value
- * of s, c0 the resting value of species 0, j0 the rate at rest (zero for
+ * does), about the resting state and scaled by the reaction step dt_r:
+ * dkon = dt_r k_on, dkoff = dt_r k_off, dka = dt_r k_on (tot - B0) with B0
+ * the resting value of s, dj0 = dt_r times the rate at rest (zero for
  * equilibrium initial values).  Padded to 8 values. */
 template <typename T>
 struct RateTab {
-  T kon, koff, a, c0, j0, pad[3];
+  T dkon, dkoff, dka, dj0, pad[4];
 };
 
 template <typename T>
@@ -109,6 +113,7 @@ struct DevParams {
    * buffer binds; rfast is set when every buffer of every domain binds
    * species 0 and no species is bound by two buffers of one domain */
   RateTab<T> rt[MAXD][MAXS];
+  T rc0[MAXD];   /* resting value of species 0 per domain */
   uint32_t rbound, rfast;
   double dt, h;
   FluxParams flux[MAXF];
@@ -144,6 +149,9 @@ struct LaunchCfg {
                     wrapping across rows, instead of 64 x rows tiles       */
   int yb;        /* >0: row-blocked kernel, yb rows per lane and `rows`
                     wavefronts side by side along x per block              */
+  int fuse;      /* 1: pairs of eligible steps run as one k_step2 pass     */
+  int fz;        /* planes per block of k_step2                             */
+  int fx;        /* xcd_tile map of k_step2's blocks                        */
 };
 
 /* Cross-lane moves within one 64-wide wavefront (DPP wave_shr:1 and
@@ -206,6 +214,7 @@ struct BlockTables {
   T dinit[NS][MAXD * MAXD];
   T kon[MAXS][MAXD], koff[MAXS][MAXD], ra[MAXS][MAXD], rf0[MAXS][MAXD], rj0[MAXS][MAXD];
   RateTab<T> rt[MAXD][NS];
+  T rc0[MAXD];
   uint8_t b0[MAXS][MAXD], b1[MAXS][MAXD];
   uint32_t fixed[MAXD];
 
@@ -226,7 +235,10 @@ struct BlockTables {
       b1[b][d] = P->bsp1[b][d];
     }
     for (int i = tid; i < MAXD * NS; i += nthr) rt[i / NS][i % NS] = P->rt[i / NS][i % NS];
-    if (tid < MAXD) fixed[tid] = P->fixed_mask[tid];
+    if (tid < MAXD) {
+      fixed[tid] = P->fixed_mask[tid];
+      rc0[tid] = P->rc0[tid];
+    }
   }
 };
 
@@ -270,15 +282,18 @@ __device__ inline void react_general(T nw[NS], const T uc[NS], const BlockTables
  * indices are compile-time, so no per-lane selection is needed. */
 template <typename T, int NS>
 __device__ inline void react_species_major(T nw[NS], const T uc[NS], const BlockTables<T, NS>& S, int d,
-                                           uint32_t rbound, T dtr)
+                                           uint32_t rbound)
 {
+  /* dt_r J = dt_r (j0 + k_on (a F - B (C0 + F)) - k_off B) about rest, with
+   * deviations F of species 0 and B of the buffer: three fused
+   * multiply-adds per buffer, the absolute species 0 shared */
+  const T f0 = S.rc0[d] + uc[0];
   T f = nw[0];
 #pragma unroll
   for (int s = 1; s < NS; s++) {
     if (!((rbound >> s) & 1u)) continue;
     const RateTab<T> r = S.rt[d][s];
-    const T J = r.j0 + r.kon * (r.a * uc[0] - uc[s] * (r.c0 + uc[0])) - r.koff * uc[s];
-    const T inc = dtr * J;
+    const T inc = (r.dka * uc[0] + r.dj0) - uc[s] * (r.dkon * f0 + r.dkoff);
     nw[s] += inc;
     f -= inc;
   }
@@ -533,7 +548,7 @@ k_stencil(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict
           }
         }
     }
-    if (react == 2) react_species_major<T, NS>(nw, uc, S, d, rbound, dtr);
+    if (react == 2) react_species_major<T, NS>(nw, uc, S, d, rbound);
     else react_general<T, NS>(nw, uc, S, d, nb, dmask, dtr);
     if (dirichlet && (xyface || z == 0 || !ztop)) nw[0] = T(0);   /* the resting value */
     const uint32_t fm = S.fixed[d];
@@ -715,7 +730,7 @@ k_stencil_lds(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __rest
         }
         nw[s] = acc;
       }
-      if (react == 2) react_species_major<T, NS>(nw, uc, S, d, rbound, dtr);
+      if (react == 2) react_species_major<T, NS>(nw, uc, S, d, rbound);
       else react_general<T, NS>(nw, uc, S, d, nb, dmask, dtr);
       if (dirichlet && (xyface || z == 0 || !ztop)) nw[0] = T(0);   /* the resting value */
       const uint32_t fm = S.fixed[d];
@@ -920,7 +935,7 @@ k_stencil_rows(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __res
         }
         nw[s] = acc;
       }
-      if (react == 2) react_species_major<T, NS>(nw, uc, S, d, rbound, dtr);
+      if (react == 2) react_species_major<T, NS>(nw, uc, S, d, rbound);
       else react_general<T, NS>(nw, uc, S, d, nb, dmask, dtr);
       if (dirichlet && (xface || yr == 0 || yr + 1 == g.ny || z == 0 || z == ztop_plane)) nw[0] = T(0);   /* the resting value */
       const uint32_t fm = S.fixed[d];
@@ -945,9 +960,61 @@ k_stencil_rows(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __res
   }
 }
 
-/* Channel gating: one block, channels strided over threads, fluxes in
- * gating order.  Ca is read from species slot 0 at the channel's cleft
- * voxel (species.c:2569). */
+/* State of channel ch of discrete flux F after the gating step at time
+ * index ti (evaluate_RyR_stochastically, boundaryfluxes.c:295-359): a
+ * channel with a forced closing time (-C) closes once that time falls
+ * within the gating interval; a stochastic one opens (closes) when its
+ * uniform is at most dt_s times the opening (closing) rate at the absolute
+ * Ca concentration c0 of its cleft voxel. */
+__device__ inline uint8_t gate_one(const FluxParams& F, uint8_t st, double c0, uint64_t ti, double t, double dts,
+                                   uint32_t seed, uint32_t instance, const double* __restrict__ uniforms,
+                                   uint64_t uniform_base, uint32_t ch)
+{
+  const double* p = F.p;
+  if (!F.stochastic) {
+    if (p[8] <= t + dts) st = 0;
+  } else {
+    const double kc = cs_rate_close(c0, p);
+    const double ko = cs_rate_open(c0, p);
+    const double r = uniforms ? uniforms[uniform_base + F.uniform_off + ch]
+                              : cs_gate_uniform(seed, instance, ti, F.group, ch);
+    if (st == 0) { if (r <= dts * ko) st = 1; }
+    else if (st == 1) { if (r <= dts * kc) st = 0; }
+  }
+  return st;
+}
+
+/* Gate every channel of every discrete flux (fluxes in gating order,
+ * channels strided over the block's threads) at step ti.  ca(to, gp)
+ * gives the stored value of species 0 at the cleft voxel `to` of global
+ * pair gp (species.c:2569). */
+template <typename T, typename Ca>
+__device__ inline void gate_channels(const DevParams<T>* __restrict__ P, const uint8_t* __restrict__ ids,
+                                     const uint32_t* __restrict__ pairs, uint8_t* __restrict__ states,
+                                     uint8_t* __restrict__ recruited, uint64_t ti, uint32_t D, uint32_t seed,
+                                     uint32_t instance, const double* __restrict__ uniforms,
+                                     uint64_t uniform_base, Ca ca)
+{
+  const double t = (double)ti * P->dt;
+  const double dts = P->dt * D;
+  const uint32_t nf = P->ndiscrete;
+  for (uint32_t f = 0; f < nf; f++) {
+    const FluxParams& F = P->flux[f];
+    for (uint32_t ch = threadIdx.x; ch < F.npairs; ch += blockDim.x) {
+      uint8_t st = states[F.state_off + ch];
+      double c0 = 0.0;
+      if (F.stochastic) {
+        const uint32_t to = pairs[2 * (F.pair_off + ch) + 1];
+        c0 = ca(to, F.pair_off + ch) + P->init64[ids[to] & 7][0];
+      }
+      st = gate_one(F, st, c0, ti, t, dts, seed, instance, uniforms, uniform_base, ch);
+      states[F.state_off + ch] = st;
+      if (st) recruited[F.state_off + ch] = 1;
+    }
+  }
+}
+
+/* Channel gating: one block (gate_channels) with Ca from u. */
 template <typename T>
 __global__ void __launch_bounds__(GATE_THREADS)
 k_gate(const T* __restrict__ u, const uint8_t* __restrict__ ids, Geo g, const uint32_t* __restrict__ pairs,
@@ -957,78 +1024,426 @@ k_gate(const T* __restrict__ u, const uint8_t* __restrict__ ids, Geo g, const ui
        uint64_t uniform_base)
 {
   const uint64_t ti = (ti_ptr ? *ti_ptr : 0) + ti_arg;
-  const double t = (double)ti * P->dt;
-  const double dts = P->dt * D;
-  const uint32_t nf = P->ndiscrete;
-  for (uint32_t f = 0; f < nf; f++) {
-    const FluxParams& F = P->flux[f];
-    const double* p = F.p;
-    for (uint32_t ch = threadIdx.x; ch < F.npairs; ch += blockDim.x) {
-      uint8_t st = states[F.state_off + ch];
-      if (!F.stochastic) {
-        if (p[8] <= t + dts) st = 0;
-      } else {
-        const uint32_t to = pairs[2 * (F.pair_off + ch) + 1];
-        const double c0 = (double)u[elem(g, 0, to)] + P->init64[ids[to] & 7][0];
-        const double kc = cs_rate_close(c0, p);
-        const double ko = cs_rate_open(c0, p);
-        const double r = uniforms ? uniforms[uniform_base + F.uniform_off + ch]
-                                  : cs_gate_uniform(seed, instance, ti, F.group, ch);
-        if (st == 0) { if (r <= dts * ko) st = 1; }
-        else if (st == 1) { if (r <= dts * kc) st = 0; }
-      }
-      states[F.state_off + ch] = st;
-      if (st) recruited[F.state_off + ch] = 1;
-    }
-  }
+  gate_channels<T>(P, ids, pairs, states, recruited, ti, D, seed, instance, uniforms, uniform_base,
+                   [&](uint32_t to, uint32_t) { return (double)u[elem(g, 0, to)]; });
 }
 
-/* Boundary fluxes gathered per (species, voxel) row. */
-template <typename T>
-__global__ void __launch_bounds__(FLUX_THREADS)
-k_flux(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__ ids,
-       const uint32_t* __restrict__ row_voxel, const uint8_t* __restrict__ row_species,
-       const uint32_t* __restrict__ row_start, const uint32_t* __restrict__ ent_pair,
-       const uint8_t* __restrict__ ent_flux, const int8_t* __restrict__ ent_sign,
-       const uint32_t* __restrict__ pairs, const uint8_t* __restrict__ states,
-       const DevParams<T>* __restrict__ P, Geo g, uint32_t nrows, uint32_t fluxmask)
+/* Flux of one boundary pair (species.c:1704-1807) from the stored
+ * deviations vf, vt at its from and to voxels and their resting values
+ * r0, r1: the RyR two-voxel relaxation (ryr_flux, boundaryfluxes.c:87-141)
+ * or the SERCA pump (serca_flux, boundaryfluxes.c:12-85). */
+__device__ inline double flux_j(const FluxParams& F, double vf, double vt, double r0, double r1, double dt,
+                                double h)
 {
-  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrows) return;
-  const uint32_t v = row_voxel[r];
-  const uint32_t s = row_species[r];
+  if (F.kind == CS_FLUX_RYR) return ((vf - vt) + (r0 - r1)) * F.half_1mK0;
+  return cs_serca_flux_d(dt, h, vf + r0, vt + r1, F.density);
+}
+
+/* The (species, voxel) rows of the boundary fluxes: row r lists, in a
+ * fixed order, every pair touching its voxel with the sign of its
+ * contribution (+ at the `to` voxel, - at an inner `from` voxel). */
+struct FluxRows {
+  const uint32_t* voxel;
+  const uint8_t* species;
+  const uint32_t* start;      /* [n + 1] */
+  const uint32_t* ent_pair;   /* pair index within its flux */
+  const uint8_t* ent_flux;
+  const int8_t* ent_sign;
+  uint32_t n;
+};
+
+/* Sum of the fluxes of row r over the fluxes in fluxmask, closed channels
+ * skipped; val(s, voxel, gp, which) gives the stored value of species s
+ * at the from (which 0) or to (which 1) voxel of global pair gp.  False
+ * when the row takes nothing: no entry applies, its species is fixed in
+ * the voxel's domain, or it is species 0 on a Dirichlet face. */
+template <typename T, typename Val>
+__device__ inline bool row_sum(const DevParams<T>* __restrict__ P, const uint8_t* __restrict__ ids, const Geo& g,
+                               const FluxRows& rows, uint32_t r, const uint32_t* __restrict__ pairs,
+                               const uint8_t* __restrict__ states, uint32_t fluxmask, double& sum, Val val)
+{
+  const uint32_t v = rows.voxel[r];
+  const uint32_t s = rows.species[r];
   const int d = ids[v] & 7;
-  if ((P->fixed_mask[d] >> s) & 1u) return;
+  if ((P->fixed_mask[d] >> s) & 1u) return false;
   if (s == 0 && P->dirichlet) {
     const uint32_t x = v % g.nx, y = (v / g.nx) % g.ny, z = (uint32_t)(v / g.plane);
     if (x == 0 || y == 0 || z == 0 || x + 1 == (uint32_t)g.nx || y + 1 == (uint32_t)g.ny ||
         z + 1 == (uint32_t)g.nz)
-      return;
+      return false;
   }
   const double dt = P->dt, h = P->h;
-  double sum = 0.0;
   bool any = false;
-  for (uint32_t e = row_start[r]; e < row_start[r + 1]; e++) {
-    const uint32_t f = ent_flux[e];
+  sum = 0.0;
+  for (uint32_t e = rows.start[r]; e < rows.start[r + 1]; e++) {
+    const uint32_t f = rows.ent_flux[e];
     if (!((fluxmask >> f) & 1u)) continue;
     const FluxParams& F = P->flux[f];
-    const uint32_t p = ent_pair[e];
+    const uint32_t p = rows.ent_pair[e];
     if (F.discrete && !states[F.state_off + p]) continue;
-    const uint32_t from = pairs[2 * (F.pair_off + p)], to = pairs[2 * (F.pair_off + p) + 1];
-    /* absolute values: stored deviation + the voxel's resting value */
-    const double r0 = P->init64[ids[from] & 7][s], r1 = P->init64[ids[to] & 7][s];
-    const double u0 = (double)u[elem(g, s, from)] + r0, u1 = (double)u[elem(g, s, to)] + r1;
-    double J;
-    if (F.kind == CS_FLUX_RYR)
-      J = (((double)u[elem(g, s, from)] - (double)u[elem(g, s, to)]) + (r0 - r1)) * F.half_1mK0;
-    else
-      J = cs_serca_flux_d(dt, h, u0, u1, F.density);
-    sum += ent_sign[e] > 0 ? J : -J;
+    const uint32_t gp = F.pair_off + p;
+    const uint32_t from = pairs[2 * gp], to = pairs[2 * gp + 1];
+    const double J = flux_j(F, val(s, from, gp, 0u), val(s, to, gp, 1u), P->init64[ids[from] & 7][s],
+                            P->init64[ids[to] & 7][s], dt, h);
+    sum += rows.ent_sign[e] > 0 ? J : -J;
     any = true;
   }
-  if (any) {
-    T* o = un + elem(g, s, v);
+  return any;
+}
+
+/* Boundary fluxes gathered per (species, voxel) row and added to the
+ * next state un, from the state u. */
+template <typename T>
+__global__ void __launch_bounds__(FLUX_THREADS)
+k_flux(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__ ids, FluxRows rows,
+       const uint32_t* __restrict__ pairs, const uint8_t* __restrict__ states,
+       const DevParams<T>* __restrict__ P, Geo g, uint32_t fluxmask)
+{
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows.n) return;
+  double sum;
+  if (row_sum<T>(P, ids, g, rows, r, pairs, states, fluxmask, sum,
+                 [&](uint32_t s, uint32_t vox, uint32_t, uint32_t) { return (double)u[elem(g, s, vox)]; })) {
+    T* o = un + elem(g, rows.species[r], rows.voxel[r]);
     *o = (T)((double)*o + sum);
+  }
+}
+
+/* ------------------------------------------------ fused two-step pass */
+
+/* Voxels of the boundary pairs for the fused pass, ordered by plane
+ * column (y*nx + x) and then z, so that a lane marching one column meets
+ * them in order.  Pair voxel k keeps its level-1 value of species 0 in
+ * slot k; row[k] is its k_flux row (every flux acts on species 0), or
+ * NO_ROW. */
+struct F2Voxels {
+  const uint32_t* col_start;   /* [plane + 1] first pair voxel of each column */
+  const uint32_t* z;           /* plane of pair voxel k                        */
+  const uint32_t* row;         /* k_flux row of pair voxel k, or NO_ROW         */
+};
+constexpr uint32_t NO_ROW = 0xFFFFFFFFu;
+
+/* What one of the two steps of a fused pass does. */
+struct F2Step {
+  uint32_t dmask, react, fluxmask, gate;
+  uint64_t ti, uni_base;
+};
+
+/* Two time steps in one pass over HBM.  A block owns a 64 x WY tile of
+ * plane columns, of which the inner (64-2) x (WY-2) are its own: every
+ * lane computes step 1 at plane z (level 1) for its column, the ring of
+ * one column around the inner tile included, and the inner lanes then
+ * compute step 2 at plane z-1 (level 2) from level-1 values, so the
+ * intermediate state never leaves the chip.  Level 0 comes from HBM with
+ * two planes in flight ahead of use: the x neighbours by DPP (the first
+ * and last lane load their outer one) and the y neighbours by loads.
+ * Level-1 values reach the x neighbours by DPP and the y neighbours
+ * through a double-buffered LDS plane (one barrier per plane); z
+ * neighbours sit in register rings.  Each
+ * level is k_stencil's per-voxel update.  The boundary fluxes of step 1,
+ * summed per k_flux row before the pass (k_bridge), are added at level 1
+ * as k_flux adds them, and the level-1 value of species 0 at every pair
+ * voxel is kept in its slot: step 2's gating and fluxes need only those
+ * values, and its fluxes add to the result after the pass (k_bridge),
+ * since no other voxel's step-2 update reads them. */
+template <typename T, int NS, int WY, bool NT>
+__global__ void __launch_bounds__(BX * WY)
+k_step2(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__ ids,
+        const DevParams<T>* __restrict__ P, Geo g, F2Voxels fv, const double* __restrict__ fsum,
+        const uint8_t* __restrict__ fany, T* __restrict__ l1, F2Step s1, F2Step s2, int zchunk, int tx, int ty,
+        int nblocks, int xcd)
+{
+  __shared__ BlockTables<T, NS> S;
+  __shared__ T ex[2][NS][WY][BX];
+
+  const int bid = xcd_tile(blockIdx.x, nblocks, xcd);
+  if (bid >= nblocks) return;   /* whole padding blocks only: no barrier is skipped */
+  const int bxi = bid % tx, byi = (bid / tx) % ty, bzi = bid / (tx * ty);
+  const int lane = threadIdx.x, wy = threadIdx.y;
+  S.load(P, wy * BX + lane, BX * WY);
+
+  const int nz = g.nz;
+  const int x = bxi * (BX - 2) - 1 + lane, y = byi * (WY - 2) - 1 + wy;
+  const bool ingrid = x >= 0 && x < g.nx && y >= 0 && y < g.ny;
+  const bool inner = ingrid && lane >= 1 && lane < BX - 1 && wy >= 1 && wy < WY - 1;
+  /* lanes outside the grid read the clamped column and store nothing: at
+   * level 0 they mirror the face column, so the DPP neighbour of a face
+   * lane is the voxel itself (no flux through the face); at level 2 the
+   * face lanes select their own value instead */
+  const int xc = min(max(x, 0), g.nx - 1), yc = min(max(y, 0), g.ny - 1);
+  const int z0 = bzi * zchunk, z1 = min(nz, z0 + zchunk);
+
+  const uint32_t ES = (uint32_t)sizeof(T);
+  const uint32_t col = (uint32_t)yc * (uint32_t)g.nx + (uint32_t)xc;
+  const uint32_t bc = col * ES;
+  const uint32_t bxm = (xc > 0 ? col - 1 : col) * ES, bxp = (xc + 1 < g.nx ? col + 1 : col) * ES;
+  const uint32_t bym = (yc > 0 ? col - g.nx : col) * ES, byp = (yc + 1 < g.ny ? col + g.nx : col) * ES;
+  const bool has_xm = x > 0, has_xp = x + 1 < g.nx;
+  const bool xyface = xc == 0 || yc == 0 || xc + 1 == g.nx || yc + 1 == g.ny;
+  /* the first and last lane load their outer x neighbour at level 0 */
+  const bool edge_lane = (lane == 0 && xc > 0) || (lane == BX - 1 && xc + 1 < g.nx);
+  const uint32_t bedge = lane == 0 ? bxm : bxp;
+  /* level 2: whether this wavefront holds a lane on an x face */
+  const bool xclamp = __ballot(ingrid && (!has_xm || !has_xp)) != 0;
+  /* LDS rows of the y neighbours (the voxel itself at the grid's y faces;
+   * the tile's outer rows use their own row and are never inner) */
+  const int wym = (y > 0 && wy > 0) ? wy - 1 : wy, wyp = (y + 1 < g.ny && wy + 1 < WY) ? wy + 1 : wy;
+
+  /* level 0 spans planes [z0-2, z1+2) of u; the host keeps (zchunk + 4)
+   * groups below 2^32 bytes */
+  const int zb = max(z0 - 2, 0), ze = min(nz, z1 + 2);
+  const size_t plane = g.plane;
+  const uint32_t pbytes = (uint32_t)plane * ES, gbytes = (uint32_t)NS * pbytes;
+  const Rsrc rin = make_rsrc(u + (size_t)zb * NS * plane, (uint32_t)(ze - zb) * gbytes);
+  const Rsrc rout = make_rsrc(un + (size_t)z0 * NS * plane, (uint32_t)(z1 - z0) * gbytes);
+  const Rsrc rid = make_rsrc(ids + (size_t)zb * plane, (uint32_t)(ze - zb) * (uint32_t)plane);
+  auto gofs = [&](int zz) { return (uint32_t)(min(max(zz, zb), ze - 1) - zb) * gbytes; };
+  auto iofs = [&](int zz) { return (uint32_t)(min(max(zz, zb), ze - 1) - zb) * (uint32_t)plane; };
+
+  const uint32_t react_nb = (uint32_t)P->nb;
+  const uint32_t rbound = P->rbound;
+  const T dtr = P->dt_react;
+  const bool dirichlet = P->dirichlet != 0;
+
+  /* pair voxels of this column from the first level-1 plane on */
+  const int zs = max(z0 - 1, 0);
+  uint32_t pk = 0, pend = 0;
+  if (ingrid) {
+    pk = fv.col_start[col];
+    pend = fv.col_start[col + 1];
+  }
+  while (pk < pend && (int)fv.z[pk] < zs) pk++;
+  int pz = pk < pend ? (int)fv.z[pk] : 0x7FFFFFFF;
+
+  /* One level at plane zz: vm, vc, vp are the level's input at planes
+   * zz-1, zz, zz+1 of this column, ym/yp its y neighbours at zz; the
+   * result (the next level at zz) goes to nw. */
+  auto level = [&](uint32_t dmask, uint32_t react, const bool first, const int zz, const uint32_t id,
+                   const T (&vm)[NS], const T (&vc)[NS], const T (&vp)[NS], const T (&ym)[NS], const T (&yp)[NS],
+                   const T (&xe)[NS], T (&nw)[NS]) {
+    const int d = id & 7;
+    const bool uni = (id & 0x80) != 0;
+    const bool ztop = zz < nz - 1;
+    const uint32_t szi = iofs(zz);
+#pragma unroll
+    for (int s = 0; s < NS; s++) nw[s] = vc[s];
+    T nxm[NS], nxp[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+      if ((dmask >> s) & 1u) {
+        nxm[s] = dpp_from_left(vc[s], xe[s]);
+        nxp[s] = dpp_from_right(vc[s], xe[s]);
+        if (!first && xclamp) {
+          nxm[s] = has_xm ? nxm[s] : vc[s];
+          nxp[s] = has_xp ? nxp[s] : vc[s];
+        }
+      }
+    if (__ballot(!uni) == 0) {
+#pragma unroll
+      for (int s = 0; s < NS; s++)
+        if ((dmask >> s) & 1u) {
+          const T c = vc[s];
+          const T exm = nxm[s] - c, exp_ = nxp[s] - c;
+          const T eym = ym[s] - c, eyp = yp[s] - c;
+          const T ezm = vm[s] - c, ezp = vp[s] - c;
+          nw[s] = c + S.coef[s][d * MAXD + d] * ((ezm + ezp) + (eym + eyp) + (exm + exp_));
+        }
+    } else {
+      int dxm = d, dxp = d, dym = d, dyp = d, dzm = d, dzp = d;
+      if (!uni) {
+        dxm = bload_u8(rid, bxm / ES, szi) & 7;
+        dxp = bload_u8(rid, bxp / ES, szi) & 7;
+        dym = bload_u8(rid, bym / ES, szi) & 7;
+        dyp = bload_u8(rid, byp / ES, szi) & 7;
+        dzm = zz > 0 ? bload_u8(rid, col, iofs(zz - 1)) & 7 : d;
+        dzp = ztop ? bload_u8(rid, col, iofs(zz + 1)) & 7 : d;
+      }
+#pragma unroll
+      for (int s = 0; s < NS; s++)
+        if ((dmask >> s) & 1u) {
+          const T c = vc[s];
+          const T exm = nxm[s] - c, exp_ = nxp[s] - c;
+          const T eym = ym[s] - c, eyp = yp[s] - c;
+          const T ezm = vm[s] - c, ezp = vp[s] - c;
+          const T* C = S.coef[s] + d * MAXD;
+          if (uni) {
+            nw[s] = c + C[d] * ((ezm + ezp) + (eym + eyp) + (exm + exp_));
+          } else {
+            const T* Dn = S.dinit[s] + d * MAXD;
+            nw[s] = c + ((C[dzm] * (ezm + Dn[dzm]) + C[dzp] * (ezp + Dn[dzp])) +
+                         (C[dym] * (eym + Dn[dym]) + C[dyp] * (eyp + Dn[dyp])) +
+                         (C[dxm] * (exm + Dn[dxm]) + C[dxp] * (exp_ + Dn[dxp])));
+          }
+        }
+    }
+    if (react == 2) {
+      react_species_major<T, NS>(nw, vc, S, d, rbound);
+    } else if (react == 1) {
+      react_general<T, NS>(nw, vc, S, d, (int)react_nb, dmask, dtr);
+    }
+    if (dirichlet && (xyface || zz == 0 || !ztop)) nw[0] = T(0);   /* the resting value */
+    const uint32_t fm = S.fixed[d];
+    if (fm) {
+#pragma unroll
+      for (int s = 0; s < NS; s++)
+        if ((fm >> s) & 1u) nw[s] = T(0);
+    }
+    /* level 1 at a pair voxel: step 1's flux sum, then the slot */
+    if (first && pz == zz) {
+      const uint32_t r = fv.row[pk];
+      if (r != NO_ROW && fany[r]) nw[0] = (T)((double)nw[0] + fsum[r]);
+      if (inner && zz >= z0 && zz < z1) l1[pk] = nw[0];
+      pk++;
+      pz = pk < pend ? (int)fv.z[pk] : 0x7FFFFFFF;
+    }
+  };
+
+  /* register rings, rotated by renaming over their common period of six
+   * planes: level 0 holds planes z-1 .. z+3 and receives z+4; level 1
+   * holds planes z-2, z-1 and receives z.  The y (and edge-lane x)
+   * neighbours of level 0 at plane z are reloaded for z+1 as soon as
+   * level 1 has used them.  Values of species a level does not diffuse
+   * are never read from the neighbour registers. */
+  T A0[NS], A1[NS], A2[NS], A3[NS], A4[NS], A5[NS];
+  T Ym[NS], Yp[NS], Xe[NS];
+  T B0[NS], B1[NS], B2[NS];
+  uint32_t I0, I1, I2, I3, I4, I5 = 0;
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    A0[s] = bload<T>(rin, bc, gofs(zs - 1) + s * pbytes);
+    A1[s] = bload<T>(rin, bc, gofs(zs) + s * pbytes);
+    A2[s] = bload<T>(rin, bc, gofs(zs + 1) + s * pbytes);
+    A3[s] = bload<T>(rin, bc, gofs(zs + 2) + s * pbytes);
+    A4[s] = bload<T>(rin, bc, gofs(zs + 3) + s * pbytes);
+    A5[s] = A1[s];
+    Ym[s] = Yp[s] = Xe[s] = A1[s];
+    if ((s1.dmask >> s) & 1u) {
+      Ym[s] = bload<T>(rin, bym, gofs(zs) + s * pbytes);
+      Yp[s] = bload<T>(rin, byp, gofs(zs) + s * pbytes);
+      if (edge_lane) Xe[s] = bload<T>(rin, bedge, gofs(zs) + s * pbytes);
+    }
+  }
+  I0 = bload_u8(rid, col, iofs(zs - 1));
+  I1 = bload_u8(rid, col, iofs(zs));
+  I2 = bload_u8(rid, col, iofs(zs + 1));
+  I3 = bload_u8(rid, col, iofs(zs + 2));
+  I4 = bload_u8(rid, col, iofs(zs + 3));
+  __builtin_amdgcn_s_waitcnt(0);
+#pragma unroll
+  for (int s = 0; s < NS; s++) B0[s] = B1[s] = B2[s] = A1[s];
+
+  /* One plane: level 1 at z from level 0 (am, ac, ap = planes z-1, z,
+   * z+1; ym, yp, xe its neighbours at z), then level 2 at z-1 from level 1
+   * (bm, bc, bn = planes z-2, z-1, z), both clamped at the grid's z faces
+   * (plane nz is plane nz-1 again). */
+  auto iter = [&](const int z, T (&am)[NS], T (&ac)[NS], T (&ap)[NS], T (&q0)[NS], T (&q1)[NS], T (&qn)[NS],
+                  T (&wm)[NS], T (&wc)[NS], T (&wn)[NS], uint32_t& idm, uint32_t& ic, uint32_t& iqn) {
+    (void)q0;
+    (void)q1;
+    __syncthreads();
+    if (z < nz) {
+      /* in flight: plane z+4, then the level-0 neighbours of plane z+1 */
+#pragma unroll
+      for (int s = 0; s < NS; s++) qn[s] = bload<T>(rin, bc, gofs(z + 4) + s * pbytes);
+      iqn = bload_u8(rid, col, iofs(z + 4));
+      level(s1.dmask, s1.react, true, z, ic, am, ac, ap, Ym, Yp, Xe, wn);
+#pragma unroll
+      for (int s = 0; s < NS; s++)
+        if ((s1.dmask >> s) & 1u) {
+          Ym[s] = bload<T>(rin, bym, gofs(z + 1) + s * pbytes);
+          Yp[s] = bload<T>(rin, byp, gofs(z + 1) + s * pbytes);
+          if (edge_lane) Xe[s] = bload<T>(rin, bedge, gofs(z + 1) + s * pbytes);
+        }
+#pragma unroll
+      for (int s = 0; s < NS; s++)
+        if ((s2.dmask >> s) & 1u) ex[z & 1][s][wy][lane] = wn[s];
+    } else {
+#pragma unroll
+      for (int s = 0; s < NS; s++) wn[s] = wc[s];
+    }
+    const int zo = z - 1;
+    if (zo >= z0) {
+      T Lm[NS], Lp[NS], nw[NS];
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        Lm[s] = Lp[s] = wc[s];
+        if ((s2.dmask >> s) & 1u) {
+          Lm[s] = ex[zo & 1][s][wym][lane];
+          Lp[s] = ex[zo & 1][s][wyp][lane];
+        }
+      }
+      level(s2.dmask, s2.react, false, zo, idm, wm, wc, wn, Lm, Lp, wc, nw);
+      if (inner) {
+        const uint32_t so = (uint32_t)(zo - z0) * gbytes;
+#pragma unroll
+        for (int s = 0; s < NS; s++) bstore<T>(rout, bc, so + s * pbytes, nw[s], NT);
+      }
+    }
+    /* at z = 0 the slot that holds plane z-2 next is plane 0 (clamped) */
+    if (z == 0) {
+#pragma unroll
+      for (int s = 0; s < NS; s++) wc[s] = wn[s];
+    }
+  };
+
+  const int zt = z1 == nz ? nz : z1;
+  for (int z = zs; z <= zt; z += 6) {
+    iter(z, A0, A1, A2, A3, A4, A5, B0, B1, B2, I0, I1, I5);
+    if (z + 1 <= zt) iter(z + 1, A1, A2, A3, A4, A5, A0, B1, B2, B0, I1, I2, I0);
+    if (z + 2 <= zt) iter(z + 2, A2, A3, A4, A5, A0, A1, B2, B0, B1, I2, I3, I1);
+    if (z + 3 <= zt) iter(z + 3, A3, A4, A5, A0, A1, A2, B0, B1, B2, I3, I4, I2);
+    if (z + 4 <= zt) iter(z + 4, A4, A5, A0, A1, A2, A3, B1, B2, B0, I4, I5, I3);
+    if (z + 5 <= zt) iter(z + 5, A5, A0, A1, A2, A3, A4, B2, B0, B1, I5, I0, I4);
+  }
+}
+
+/* Channel gating and boundary fluxes around fused passes, in one block
+ * with barriers between the phases:
+ *   tail, after a pass over steps t, t+1: k_gate and k_flux of step t+1,
+ *     with the values of species 0 at the pair voxels taken from the
+ *     pass's level-1 slots, the fluxes added to u (the state after t+1);
+ *   head, before a pass over steps t', t'+1: k_gate of step t' on u, then
+ *     the flux sum of every k_flux row from u, which the pass adds at
+ *     level 1.
+ * Either part may be off.  The same gate_channels / row_sum code as
+ * k_gate / k_flux gives the same results. */
+template <typename T>
+__global__ void __launch_bounds__(BRIDGE_THREADS)
+k_bridge(T* __restrict__ u, const uint8_t* __restrict__ ids, const DevParams<T>* __restrict__ P, Geo g,
+         const uint32_t* __restrict__ pairs, const uint32_t* __restrict__ pslot, const T* __restrict__ l1,
+         uint8_t* __restrict__ states, uint8_t* __restrict__ recruited, FluxRows rows, double* __restrict__ fsum,
+         uint8_t* __restrict__ fany, F2Step tail, uint32_t do_tail, F2Step head, uint32_t do_head, uint32_t D,
+         uint32_t seed, uint32_t instance, const double* __restrict__ uniforms)
+{
+  if (do_tail) {
+    if (tail.gate)
+      gate_channels<T>(P, ids, pairs, states, recruited, tail.ti, D, seed, instance, uniforms, tail.uni_base,
+                       [&](uint32_t, uint32_t gp) { return (double)l1[pslot[2 * gp + 1]]; });
+    __syncthreads();
+    if (tail.fluxmask)
+      for (uint32_t r = threadIdx.x; r < rows.n; r += blockDim.x) {
+        double sum = 0.0;
+        const uint32_t v = rows.voxel[r];
+        if (row_sum<T>(P, ids, g, rows, r, pairs, states, tail.fluxmask, sum,
+                       [&](uint32_t, uint32_t, uint32_t gp, uint32_t which) { return (double)l1[pslot[2 * gp + which]]; })) {
+          T* o = u + elem(g, 0, v);
+          *o = (T)((double)*o + sum);
+        }
+      }
+    __syncthreads();
+  }
+  if (do_head) {
+    if (head.gate)
+      gate_channels<T>(P, ids, pairs, states, recruited, head.ti, D, seed, instance, uniforms, head.uni_base,
+                       [&](uint32_t to, uint32_t) { return (double)u[elem(g, 0, to)]; });
+    __syncthreads();
+    for (uint32_t r = threadIdx.x; r < rows.n; r += blockDim.x) {
+      double sum = 0.0;
+      const bool any = row_sum<T>(P, ids, g, rows, r, pairs, states, head.fluxmask, sum,
+                                  [&](uint32_t s, uint32_t vox, uint32_t, uint32_t) { return (double)u[elem(g, s, vox)]; });
+      fsum[r] = sum;
+      fany[r] = any ? 1 : 0;
+    }
   }
 }
 
@@ -1245,9 +1660,20 @@ struct cs_context {
   std::vector<uint8_t> init_states;
   uint32_t substeps[MAXS] = {0};
   uint32_t flux_substep[MAXF] = {0};
-  LaunchCfg launch{4, 32, 1, 1, 0, 0, 0, 0};
+  LaunchCfg launch{4, 32, 1, 1, 0, 0, 0, 0, 0, 32, 1};
   uint32_t rfast = 0;   /* buffer reactions admit the species-major form */
   std::vector<uint8_t> h_ids;   /* domain ids in device order (host copy)  */
+  /* fused two-step pass (k_step2) */
+  uint32_t* d_f2_col = nullptr;      /* pair voxels by column, see F2Voxels   */
+  uint32_t* d_f2_z = nullptr;
+  uint32_t* d_f2_row = nullptr;
+  uint32_t* d_f2_pslot = nullptr;    /* slots of each pair's from and to voxel */
+  void* d_f2_l1 = nullptr;           /* level-1 species 0 at the pair voxels  */
+  double* d_f2_fsum = nullptr;       /* step-1 flux sum per k_flux row        */
+  uint8_t* d_f2_fany = nullptr;      /* the row takes its sum                 */
+  bool f2_ok = false;                /* the problem admits fused passes       */
+  std::string f2_why;                /* why not, when it does not             */
+  uint32_t fsm = 0;                  /* species the fluxes act on (bit 0)     */
 };
 
 namespace {
@@ -1269,14 +1695,24 @@ int max_zchunk(const Geo& g, size_t esize)
   return (int)std::min<uint64_t>((uint64_t)g.nz, ((1ull << 32) - 1) / group - 3);
 }
 
+/* Largest z chunk of k_step2, whose blocks span (chunk + 4) z-groups
+ * under 32-bit descriptor offsets (0 when not even one plane fits). */
+int max_fused_zchunk(const Geo& g, size_t esize)
+{
+  const uint64_t group = g.plane * esize * g.ns;
+  const uint64_t fit = ((1ull << 32) - 1) / group;
+  return fit > 4 ? (int)std::min<uint64_t>((uint64_t)g.nz, fit - 4) : 0;
+}
+
 /* Default launch shape, the fastest measured on MI355X for the 32M-voxel
  * single dyad in FP32: linear columns, blocks of 4 wavefronts, 8 planes
  * per block, groups of 16 tiles per XCD, one plane ahead, non-temporal
  * stores of the next state. */
 LaunchCfg default_launch(const Geo& g, size_t esize)
 {
-  LaunchCfg L{4, 8, 16, 1, 1, 0, 1, 0};
+  LaunchCfg L{4, 8, 16, 1, 1, 0, 1, 0, 0, 50, 1};
   L.zchunk = std::min(L.zchunk, max_zchunk(g, esize));
+  L.fz = std::max(1, std::min(L.fz, max_fused_zchunk(g, esize)));
   return L;
 }
 
@@ -1287,7 +1723,8 @@ void free_ctx(cs_context* c)
   void* ptrs[] = {c->d_u[0], c->d_u[1], c->d_ids, c->d_params, c->d_pairs, c->d_states, c->d_recruited,
                   c->d_row_voxel, c->d_row_species, c->d_row_start, c->d_ent_pair,
                   c->d_ent_flux, c->d_ent_sign, c->d_uniforms, c->d_partial, c->d_stats,
-                  c->d_lweights, c->d_lpart};
+                  c->d_lweights, c->d_lpart, c->d_f2_col, c->d_f2_z,
+                  c->d_f2_row, c->d_f2_pslot, c->d_f2_l1, c->d_f2_fsum, c->d_f2_fany};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -1346,15 +1783,16 @@ void fill_params(DevParams<T>& dp, const cs_problem* p, const cs_context* c)
       seen |= 1u << b0;
       const double kon = p->k_on[b][d], koff = p->k_off[b][d];
       const double B0 = p->init[d][b0], C0 = p->init[d][0], a = p->tot[b][d] - B0;
-      dp.rt[d][b0].kon = (T)kon;
-      dp.rt[d][b0].koff = (T)koff;
-      dp.rt[d][b0].a = (T)a;
-      dp.rt[d][b0].c0 = (T)C0;
-      dp.rt[d][b0].j0 = (T)(kon * a * C0 - koff * B0);
+      const double dtr = p->dt_react;
+      dp.rt[d][b0].dkon = (T)(dtr * kon);
+      dp.rt[d][b0].dkoff = (T)(dtr * koff);
+      dp.rt[d][b0].dka = (T)(dtr * kon * a);
+      dp.rt[d][b0].dj0 = (T)(dtr * (kon * a * C0 - koff * B0));
       dp.rbound |= 1u << b0;
     }
   }
   dp.rfast = fast ? 1u : 0u;
+  for (int d = 0; d < MAXD; d++) dp.rc0[d] = (T)p->init[d][0];
   dp.dt_react = (T)p->dt_react;
   dp.nb = p->num_buffers;
   dp.dirichlet = p->force_dirichlet;
@@ -1551,6 +1989,12 @@ void launch_stencil(cs_context* c, uint32_t dmask, uint32_t react)
   launch_stencil_cfg<T>(c, c->launch, c->d_u[c->cur], c->d_u[c->cur ^ 1], dmask, react);
 }
 
+FluxRows flux_rows(const cs_context* c)
+{
+  return FluxRows{c->d_row_voxel, c->d_row_species, c->d_row_start, c->d_ent_pair,
+                  c->d_ent_flux, c->d_ent_sign, c->nrows};
+}
+
 /* Buffer reactions of step ti: 0 none, 1 in the reference's slot order,
  * 2 in species-major form (allowed when species 0 diffuses this step). */
 uint32_t react_mode(const cs_context* c, uint64_t ti, uint32_t dmask)
@@ -1584,13 +2028,89 @@ int step_one(cs_context* c, uint64_t ti, const double* d_uni, uint64_t uni_base,
   if (c->nrows && fluxmask) {
     const uint32_t blocks = (c->nrows + FLUX_THREADS - 1) / FLUX_THREADS;
     hipLaunchKernelGGL(k_flux<T>, dim3(blocks), dim3(FLUX_THREADS), 0, c->stream, ucur,
-                       (T*)c->d_u[c->cur ^ 1], c->d_ids, c->d_row_voxel, c->d_row_species,
-                       c->d_row_start, c->d_ent_pair, c->d_ent_flux, c->d_ent_sign, c->d_pairs,
-                       c->d_states, (const DevParams<T>*)c->d_params, c->geo, c->nrows, fluxmask);
+                       (T*)c->d_u[c->cur ^ 1], c->d_ids, flux_rows(c), c->d_pairs, c->d_states,
+                       (const DevParams<T>*)c->d_params, c->geo, fluxmask);
   }
   HIP_TRY(hipGetLastError());
   c->cur ^= 1;
   return CS_OK;
+}
+
+/* Species whose diffusion substep fires at step ti. */
+uint32_t diffusing(const cs_context* c, uint64_t ti)
+{
+  uint32_t m = 0;
+  for (uint32_t s = 0; s < c->ns; s++)
+    if (c->substeps[s] && ti % c->substeps[s] == 0) m |= 1u << s;
+  return m;
+}
+
+F2Step step_info(const cs_context* c, uint64_t ti, uint64_t uni_base)
+{
+  F2Step st{};
+  st.dmask = diffusing(c, ti);
+  st.react = react_mode(c, ti, st.dmask);
+  for (uint32_t f = 0; f < c->nf; f++)
+    if (ti % c->flux_substep[f] == 0) st.fluxmask |= 1u << f;
+  st.gate = (c->ndiscrete && ti % c->prob.stochastic_substep == 0) ? 1u : 0u;
+  st.ti = ti;
+  st.uni_base = uni_base;
+  return st;
+}
+
+/* Steps ti and ti+1 may run as one k_step2 pass: the problem admits it
+ * and every flux species (species 0 too when channels gate) diffuses on
+ * both steps, so its neighbours are at hand at both levels. */
+bool can_fuse(const cs_context* c, const LaunchCfg& L, uint64_t ti)
+{
+  if (!L.fuse || !c->f2_ok) return false;
+  return (c->fsm & ~diffusing(c, ti)) == 0 && (c->fsm & ~diffusing(c, ti + 1)) == 0;
+}
+
+template <typename T, int NS>
+void launch_step2_ns(cs_context* c, const LaunchCfg& L, const void* uin, void* uout, const F2Step& a,
+                     const F2Step& b)
+{
+  constexpr int WY = sizeof(T) == 4 ? 16 : 8;
+  const Geo& g = c->geo;
+  const int tx = (g.nx + BX - 3) / (BX - 2), ty = (g.ny + WY - 3) / (WY - 2);
+  const int zc = std::max(1, std::min(L.fz, g.nz));
+  const int tz = (g.nz + zc - 1) / zc;
+  const int nblocks = tx * ty * tz;
+  const F2Voxels fv{c->d_f2_col, c->d_f2_z, c->d_f2_row};
+  hipLaunchKernelGGL((k_step2<T, NS, WY, true>), dim3(xcd_launch(nblocks, L.fx)), dim3(BX, WY), 0, c->stream,
+                     (const T*)uin, (T*)uout, c->d_ids, (const DevParams<T>*)c->d_params, g, fv,
+                     (const double*)c->d_f2_fsum, (const uint8_t*)c->d_f2_fany, (T*)c->d_f2_l1, a, b, zc, tx, ty,
+                     nblocks, L.fx);
+}
+
+template <typename T>
+void launch_step2(cs_context* c, const LaunchCfg& L, const void* uin, void* uout, const F2Step& a,
+                  const F2Step& b)
+{
+  switch (c->ns) {
+    case 1: launch_step2_ns<T, 1>(c, L, uin, uout, a, b); break;
+    case 2: launch_step2_ns<T, 2>(c, L, uin, uout, a, b); break;
+    case 3: launch_step2_ns<T, 3>(c, L, uin, uout, a, b); break;
+    case 4: launch_step2_ns<T, 4>(c, L, uin, uout, a, b); break;
+    case 5: launch_step2_ns<T, 5>(c, L, uin, uout, a, b); break;
+    case 6: launch_step2_ns<T, 6>(c, L, uin, uout, a, b); break;
+    case 7: launch_step2_ns<T, 7>(c, L, uin, uout, a, b); break;
+    default: break;   /* more species never fuse (f2_ok is false) */
+  }
+}
+
+/* k_bridge on the state u: the tail of a pass (step `tail`) and/or the
+ * head of the next (step `head`). */
+template <typename T>
+void launch_bridge(cs_context* c, void* u, const F2Step& tail, bool do_tail, const F2Step& head, bool do_head,
+                   const double* d_uni)
+{
+  hipLaunchKernelGGL(k_bridge<T>, dim3(1), dim3(BRIDGE_THREADS), 0, c->stream, (T*)u, c->d_ids,
+                     (const DevParams<T>*)c->d_params, c->geo, c->d_pairs, c->d_f2_pslot, (const T*)c->d_f2_l1,
+                     c->d_states, c->d_recruited, flux_rows(c), c->d_f2_fsum, c->d_f2_fany, tail,
+                     do_tail ? 1u : 0u, head, do_head ? 1u : 0u, (uint32_t)c->prob.stochastic_substep,
+                     (uint32_t)c->prob.seed, (uint32_t)c->prob.instance, d_uni);
 }
 
 uint64_t gates_in(const cs_context* c, uint64_t first, uint64_t n)
@@ -1605,7 +2125,8 @@ uint64_t gates_in(const cs_context* c, uint64_t first, uint64_t n)
 
 template <typename T>
 int run_steps(cs_context* c, uint64_t first, uint64_t n, const double* host_uni, uint64_t count,
-              bool timing, double* total_ms, double* kms)
+              bool timing, double* total_ms, double* kms, uint64_t* npasses = nullptr,
+              uint64_t* nfused = nullptr)
 {
   HIP_TRY(hipSetDevice(c->device));
   const double* d_uni = nullptr;
@@ -1641,14 +2162,47 @@ int run_steps(cs_context* c, uint64_t first, uint64_t n, const double* host_uni,
     for (auto& e : evs) HIP_TRY(hipEventCreate(&e));
     HIP_TRY(hipEventRecord(evs[2 * n], c->stream));
   }
-  uint64_t gate_idx = 0;
-  for (uint64_t i = 0; i < n; i++) {
+  uint64_t gate_idx = 0, passes = 0, fused = 0;
+  const uint64_t spg = c->stoch_per_gate;
+  auto gates = [c](uint64_t ti) { return c->ndiscrete && ti % c->prob.stochastic_substep == 0; };
+  /* primed: gating and flux sums of the next pass's first step are done
+   * (by the previous pass's k_bridge) */
+  bool primed = false;
+  for (uint64_t i = 0; i < n;) {
     const uint64_t ti = first + i;
-    const bool gate = c->ndiscrete && ti % c->prob.stochastic_substep == 0;
-    int rc = step_one<T>(c, ti, d_uni, gate_idx * c->stoch_per_gate,
-                         timing ? evs[2 * i] : nullptr, timing ? evs[2 * i + 1] : nullptr);
-    if (rc != CS_OK) return rc;
-    if (gate) gate_idx++;
+    hipEvent_t ea = timing ? evs[2 * passes] : nullptr, eb = timing ? evs[2 * passes + 1] : nullptr;
+    if (i + 1 < n && can_fuse(c, c->launch, ti)) {
+      if (!primed) {
+        const F2Step h = step_info(c, ti, gate_idx * spg);
+        if (gates(ti)) gate_idx++;
+        launch_bridge<T>(c, c->d_u[c->cur], h, false, h, true, d_uni);
+      }
+      const F2Step s1 = step_info(c, ti, 0);
+      const F2Step s2 = step_info(c, ti + 1, gate_idx * spg);
+      if (gates(ti + 1)) gate_idx++;
+      if (ea) HIP_TRY(hipEventRecord(ea, c->stream));
+      launch_step2<T>(c, c->launch, c->d_u[c->cur], c->d_u[c->cur ^ 1], s1, s2);
+      if (eb) HIP_TRY(hipEventRecord(eb, c->stream));
+      const bool next = i + 3 < n && can_fuse(c, c->launch, ti + 2);
+      F2Step h2 = s2;
+      if (next) {
+        h2 = step_info(c, ti + 2, gate_idx * spg);
+        if (gates(ti + 2)) gate_idx++;
+      }
+      launch_bridge<T>(c, c->d_u[c->cur ^ 1], s2, true, h2, next, d_uni);
+      HIP_TRY(hipGetLastError());
+      c->cur ^= 1;
+      primed = next;
+      i += 2;
+      fused++;
+    } else {
+      const int rc = step_one<T>(c, ti, d_uni, gate_idx * spg, ea, eb);
+      if (rc != CS_OK) return rc;
+      if (gates(ti)) gate_idx++;
+      primed = false;
+      i += 1;
+    }
+    passes++;
   }
   if (timing) {
     HIP_TRY(hipEventRecord(evs[2 * n + 1], c->stream));
@@ -1657,11 +2211,13 @@ int run_steps(cs_context* c, uint64_t first, uint64_t n, const double* host_uni,
     HIP_TRY(hipEventElapsedTime(&ms, evs[2 * n], evs[2 * n + 1]));
     *total_ms = ms;
     double ksum = 0.0;
-    for (uint64_t i = 0; i < n; i++) {
+    for (uint64_t i = 0; i < passes; i++) {
       HIP_TRY(hipEventElapsedTime(&ms, evs[2 * i], evs[2 * i + 1]));
       ksum += ms;
     }
-    *kms = n ? ksum / (double)n : 0.0;
+    *kms = passes ? ksum / (double)passes : 0.0;
+    if (npasses) *npasses = passes;
+    if (nfused) *nfused = fused;
   }
   return CS_OK;
 }
@@ -1733,6 +2289,51 @@ int build_flux_rows(cs_context* c, const cs_problem* p)
   }
   if ((rc = up(&c->d_states, c->init_states)) != CS_OK) return rc;
   if ((rc = up(&c->d_recruited, c->init_states)) != CS_OK) return rc;
+
+  /* pair voxels of the fused pass (k_step2, k_bridge) */
+  const Geo& g = c->geo;
+  c->f2_ok = true;
+  c->f2_why.clear();
+  c->fsm = 0;
+  auto why = [c](const char* m) {
+    if (c->f2_ok) { c->f2_ok = false; c->f2_why = m; }
+  };
+  if (c->ns > 7) why("more than 7 species");
+  if (max_fused_zchunk(g, c->esize) < 1) why("planes too large for 32-bit offsets");
+  for (uint32_t f = 0; f < p->num_fluxes; f++)
+    if (p->fluxes[f].num_pairs) {
+      c->fsm |= 1u << p->fluxes[f].species;
+      if (p->fluxes[f].species != 0) why("a flux acts on a species other than 0");
+    }
+  /* one block gates every channel and sums every row between passes */
+  if (c->nrows > F2_MAX_ROWS || c->init_states.size() > F2_MAX_ROWS) why("more than 16384 flux rows or channels");
+  std::vector<uint64_t> keys;
+  keys.reserve(pairs_dev.size());
+  for (uint32_t v : pairs_dev) keys.push_back((uint64_t)(v % g.plane) << 32 | (uint64_t)(v / g.plane));
+  std::sort(keys.begin(), keys.end());
+  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+  std::unordered_map<uint32_t, uint32_t> slot;
+  std::vector<uint32_t> col_start(g.plane + 1, 0), vz(keys.size()), vrow(keys.size(), NO_ROW);
+  for (size_t k = 0; k < keys.size(); k++) {
+    const uint64_t colv = keys[k] >> 32, z = keys[k] & 0xFFFFFFFFu;
+    slot[(uint32_t)(z * g.plane + colv)] = (uint32_t)k;
+    vz[k] = (uint32_t)z;
+    col_start[colv + 1]++;
+  }
+  for (size_t k = 1; k < col_start.size(); k++) col_start[k] += col_start[k - 1];
+  for (uint32_t r = 0; r < (uint32_t)row_voxel.size(); r++)
+    if (row_species[r] == 0) vrow[slot[row_voxel[r]]] = r;
+  std::vector<uint32_t> pslot(pairs_dev.size());
+  for (size_t k = 0; k < pairs_dev.size(); k++) pslot[k] = slot[pairs_dev[k]];
+  if ((rc = up(&c->d_f2_col, col_start)) != CS_OK) return rc;
+  if ((rc = up(&c->d_f2_z, vz)) != CS_OK) return rc;
+  if ((rc = up(&c->d_f2_row, vrow)) != CS_OK) return rc;
+  if ((rc = up(&c->d_f2_pslot, pslot)) != CS_OK) return rc;
+  HIP_TRY(hipMalloc(&c->d_f2_l1, c->esize * std::max<size_t>(keys.size(), 1)));
+  HIP_TRY(hipMalloc((void**)&c->d_f2_fsum, sizeof(double) * std::max<size_t>(row_voxel.size(), 1)));
+  HIP_TRY(hipMalloc((void**)&c->d_f2_fany, std::max<size_t>(row_voxel.size(), 1)));
+  /* the host vectors above end here */
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return CS_OK;
 }
 
@@ -1929,6 +2530,24 @@ int cs_synchronize(cs_context* c)
   return CS_OK;
 }
 
+int cs_fused_available(const cs_context* c, char* why, size_t why_len)
+{
+  if (!c) return set_err(CS_EINVAL, "context is NULL");
+  if (why && why_len) snprintf(why, why_len, "%s", c->f2_ok ? "" : c->f2_why.c_str());
+  return c->f2_ok ? 1 : 0;
+}
+
+int cs_time_passes(cs_context* c, uint64_t first, uint64_t n, double* total_ms, double* pass_ms,
+                   uint64_t* passes, uint64_t* fused)
+{
+  if (!c || !total_ms || !pass_ms) return set_err(CS_EINVAL, "NULL argument");
+  if (first == 0) return set_err(CS_EINVAL, "time indices start at 1");
+  if (c->prob.rng == CS_RNG_MT && c->stoch_per_gate)
+    return set_err(CS_EINVAL, "cs_time_passes needs Philox gating");
+  return c->esize == 8 ? run_steps<double>(c, first, n, nullptr, 0, true, total_ms, pass_ms, passes, fused)
+                       : run_steps<float>(c, first, n, nullptr, 0, true, total_ms, pass_ms, passes, fused);
+}
+
 int cs_time_steps(cs_context* c, uint64_t first, uint64_t n, double* total_ms, double* kernel_ms)
 {
   if (!c || !total_ms || !kernel_ms) return set_err(CS_EINVAL, "NULL argument");
@@ -2097,6 +2716,9 @@ int cs_get_launch_config(const cs_context* c, cs_launch_config* out)
   out->lds_tile = (uint32_t)c->launch.lds;
   out->linear = (uint32_t)c->launch.lin;
   out->yblock = (uint32_t)c->launch.yb;
+  out->fuse = (uint32_t)c->launch.fuse;
+  out->fuse_zchunk = (uint32_t)c->launch.fz;
+  out->fuse_xcd = (uint32_t)c->launch.fx;
   return CS_OK;
 }
 
@@ -2133,6 +2755,16 @@ int cs_set_launch_config(cs_context* c, const cs_launch_config* in)
   L.lds = (int)in->lds_tile;
   L.lin = (int)in->linear;
   L.yb = (int)in->yblock;
+  if (in->fuse > 1) return set_err(CS_EINVAL, "fuse must be 0 or 1");
+  if (in->fuse && !c->f2_ok) return set_err(CS_EINVAL, "fused passes unavailable: %s", c->f2_why.c_str());
+  if (in->fuse_xcd > 64) return set_err(CS_EINVAL, "fuse_xcd is 0, 1 or a group size up to 64");
+  const int fzmax = max_fused_zchunk(c->geo, c->esize);
+  if (in->fuse_zchunk < 1) return set_err(CS_EINVAL, "fuse_zchunk must be >= 1");
+  if (fzmax >= 1 && (int)std::min<uint32_t>(in->fuse_zchunk, (uint32_t)c->geo.nz) > fzmax)
+    return set_err(CS_EINVAL, "fuse_zchunk %u exceeds %d for this grid", in->fuse_zchunk, fzmax);
+  L.fuse = (int)in->fuse;
+  L.fz = (int)std::min<uint32_t>(in->fuse_zchunk, (uint32_t)c->geo.nz);
+  L.fx = (int)in->fuse_xcd;
   c->launch = L;
   return CS_OK;
 }
@@ -2221,6 +2853,74 @@ int cs_autotune(cs_context* c, uint64_t first_time_ind, uint32_t reps, cs_launch
       mean += (double)ms / reps * (double)q.count / (double)period;
     }
     if (mean < wms) { wms = mean; winner = L; }
+  }
+  winner.fuse = 0;
+  winner.fz = c->launch.fz;
+  winner.fx = c->launch.fx;
+  /* fused two-step passes: the pairs (ti, ti+1) of two periods that may
+   * fuse, grouped by their patterns, without gating (so the channel
+   * states and recruitment flags stay as they are); a pass counts as two
+   * steps against the best single-step shape */
+  if (c->f2_ok) {
+    struct Pair { F2Step a, b; uint64_t count; };
+    std::vector<Pair> prs;
+    uint64_t fusable = 0;
+    for (uint64_t k = 0; k < 2 * period; k += 2) {
+      const uint64_t ti = first_time_ind + k;
+      LaunchCfg on = winner;
+      on.fuse = 1;
+      if (!can_fuse(c, on, ti)) continue;
+      F2Step a = step_info(c, ti, 0), b = step_info(c, ti + 1, 0);
+      a.gate = b.gate = 0;
+      fusable++;
+      bool found = false;
+      for (auto& q : prs)
+        if (q.a.dmask == a.dmask && q.a.react == a.react && q.a.fluxmask == a.fluxmask && q.b.dmask == b.dmask &&
+            q.b.react == b.react && q.b.fluxmask == b.fluxmask) { q.count++; found = true; break; }
+      if (!found) prs.push_back({a, b, 1});
+    }
+    const int fzmax = max_fused_zchunk(c->geo, c->esize);
+    std::vector<int> fzs;
+    for (int zc : {16, 25, 32, 50, 64, 100})
+      if (zc < nz && zc <= fzmax) fzs.push_back(zc);
+    if (nz <= fzmax) fzs.push_back(nz);
+    double fms = 1e30;
+    LaunchCfg fbest = winner;
+    for (int zc : fzs)
+      for (int fx : {0, 1, 8}) {
+        LaunchCfg L = winner;
+        L.fuse = 1;
+        L.fz = zc;
+        L.fx = fx;
+        double mean = 0;
+        for (const Pair& q : prs) {
+          /* the pass and its bridge (tail and the next head, no gating) */
+          auto go = [&]() {
+            if (c->esize == 8) {
+              launch_step2<double>(c, L, c->d_u[c->cur], c->d_u[c->cur ^ 1], q.a, q.b);
+              launch_bridge<double>(c, c->d_u[c->cur ^ 1], q.b, true, q.a, true, nullptr);
+            } else {
+              launch_step2<float>(c, L, c->d_u[c->cur], c->d_u[c->cur ^ 1], q.a, q.b);
+              launch_bridge<float>(c, c->d_u[c->cur ^ 1], q.b, true, q.a, true, nullptr);
+            }
+          };
+          go();
+          if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = set_err(CS_EDEVICE, "event record failed"); goto done; }
+          for (uint32_t r = 0; r < reps; r++) go();
+          if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+            rc = set_err(CS_EDEVICE, "autotune launch failed: %s", hipGetErrorString(hipGetLastError()));
+            goto done;
+          }
+          float ms = 0;
+          hipEventElapsedTime(&ms, e0, e1);
+          mean += (double)ms / reps * (double)q.count / (double)fusable;
+        }
+        if (!prs.empty() && mean < fms) { fms = mean; fbest = L; }
+      }
+    if (!prs.empty() && fms < 2.0 * wms) {
+      winner = fbest;
+      wms = fms / 2.0;
+    }
   }
 done:
   hipEventDestroy(e0);

@@ -171,10 +171,21 @@ int cs_set_channel_states(cs_context* ctx, uint32_t flux, const uint8_t* states)
 int cs_get_channel_recruited(cs_context* ctx, uint32_t flux, uint8_t* recruited);
 
 /* Time nsteps steps on the context stream with HIP events: total wall
- * time of the range and the mean duration of the fused stencil/reaction
- * kernel launches (ms).  Uses Philox gating. */
+ * time of the range and the mean duration of the stencil/reaction passes
+ * (ms; a fused pass covers two steps).  Uses Philox gating. */
 int cs_time_steps(cs_context* ctx, uint64_t first_time_ind, uint64_t nsteps,
                   double* total_ms, double* stencil_kernel_ms);
+
+/* cs_time_steps that also reports the number of passes and how many of
+ * them were fused two-step passes. */
+int cs_time_passes(cs_context* ctx, uint64_t first_time_ind, uint64_t nsteps, double* total_ms,
+                   double* pass_ms, uint64_t* passes, uint64_t* fused);
+
+/* 1 when the problem admits fused two-step passes, else 0 with the
+ * reason in why (if given).  They need boundary pairs that are face
+ * neighbours, at most one channel per voxel, at most 7 species and fewer
+ * than 65536 planes.  No reference counterpart. */
+int cs_fused_available(const cs_context* ctx, char* why, size_t why_len);
 
 /* Launch shape of the fused stencil/reaction kernel: blocks of 64 x rows
  * threads march zchunk planes; xcd_remap orders tiles over the 8 XCDs so
@@ -189,11 +200,18 @@ int cs_time_steps(cs_context* ctx, uint64_t first_time_ind, uint64_t nsteps,
  * no lane idles when nx is not a multiple of 64; yblock > 0 selects the
  * row-blocked kernel, in which every lane owns yblock consecutive rows (2
  * or 4), a block is `rows` wavefronts side by side along x (1 to 8) and x
- * neighbours move between lanes by DPP.  No reference counterpart: the
- * reference's OpenMP/AVX-512 loop has no launch shape
- * (kernel_generic.cpp:561-600). */
+ * neighbours move between lanes by DPP.  fuse = 1 runs each pair of
+ * consecutive steps whose flux species (and species 0, with channels)
+ * diffuse on both steps as one two-step pass (k_step2): a block computes
+ * the first step on its tile plus a one-voxel ring and the second on the
+ * tile, so the intermediate state stays on chip and the fields cross HBM
+ * once per two steps; fuse_zchunk is its planes per block and fuse_xcd
+ * its tile order (as xcd_remap).  The other steps, and every step with
+ * fuse = 0, run one at a time.  No reference counterpart: the reference's
+ * OpenMP/AVX-512 loop has no launch shape (kernel_generic.cpp:561-600). */
 typedef struct cs_launch_config {
   uint32_t rows, zchunk, xcd_remap, prefetch, nontemporal, lds_tile, linear, yblock;
+  uint32_t fuse, fuse_zchunk, fuse_xcd;
 } cs_launch_config;
 
 int cs_get_launch_config(const cs_context* ctx, cs_launch_config* cfg);
@@ -203,9 +221,12 @@ int cs_set_launch_config(cs_context* ctx, const cs_launch_config* cfg);
  * fastest.  The steps from first_time_ind over one period of the substep
  * pattern are grouped by (diffusing species, reactions); each group is
  * launched reps times per candidate and weighted by its share of the
- * period.  Launches read the current state and write the spare buffer, so
- * the state is unchanged; best_ms is the winner's weighted mean kernel
- * time. */
+ * period.  When the problem admits fused passes, the fusable step pairs
+ * of two periods are timed the same way for each fused shape (without
+ * gating), and fusing is kept when a pass beats two single steps.
+ * Launches read the current state and write the spare buffer, so the
+ * state is unchanged; best_ms is the winner's weighted mean kernel time
+ * per step. */
 int cs_autotune(cs_context* ctx, uint64_t first_time_ind, uint32_t reps,
                 cs_launch_config* best, double* best_ms);
 
