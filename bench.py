@@ -10,10 +10,14 @@ fluxes).  Inputs are synthetic and resident in HBM before timing starts.
 
 Prints one JSON line.  `value` is the whole-step algorithmic HBM
 bandwidth: 41 B per voxel-step (1 B domain id + 5 species x (4 B read +
-4 B write)) x voxels / step time.  `roofline` prices the fused
-stencil/reaction kernel alone, timed with HIP events on its own stream.
-`cpu_baseline` runs the FP64 C oracle on the host cores for a bounded
-sample of the same workload.
+4 B write)) x voxels / step time.  `roofline` prices the stencil pass
+alone (HIP events on its stream): a one-step pass (k_stencil) or a fused
+two-step pass (k_step2, the default where the problem admits it), priced
+at 41 B per voxel-step times the voxel-steps one launch processes (one
+or two per voxel).  `traffic` is the HBM bytes per launch the committed
+rocprofv3 counters measured for the same launch shape.  `cpu_baseline`
+runs the FP64 C oracle on the host cores for a bounded sample of the same
+workload.
 """
 from __future__ import annotations
 
@@ -38,7 +42,8 @@ METRIC = "spark realisations/sec (closure vs plain) at KS<0.05; stencil HBM GB/s
 TSTOP_MS = 30.0
 PROFILE_TRAFFIC = os.path.join(REPO, "profiles", "traffic.json")
 WORKLOAD_KEY = "cluster_dyad"
-LAUNCH_KEYS = ["rows", "zchunk", "xcd_remap", "prefetch", "nontemporal", "lds_tile", "linear", "yblock"]
+LAUNCH_KEYS = ["rows", "zchunk", "xcd_remap", "prefetch", "nontemporal", "lds_tile", "linear", "yblock",
+               "fuse", "fuse_zchunk", "fuse_xcd"]
 
 
 def log(msg):
@@ -73,9 +78,11 @@ def main():
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--size", type=int, nargs=3, default=[400, 400, 200])
-    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time the candidate launch shapes first and keep the fastest (default: the library's "
+                         "default shape, which the committed profiles cover)")
     ap.add_argument("--launch", type=str, default="",
-                    help="fixed launch shape rows,zchunk,xcd_remap,prefetch,nontemporal,lds_tile,linear,yblock (skips autotune)")
+                    help="fixed launch shape " + ",".join(LAUNCH_KEYS) + " (a prefix of them; skips autotune)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB (for scaling traffic counters) and exit")
     args = ap.parse_args()
@@ -97,26 +104,28 @@ def main():
     nv = int(np.prod(sim.n))
     bpv = BYTES_F32 if args.precision == "f32" else BYTES_F64
 
-    sim.step(args.warmup)
-    sim.synchronize()
     launch = sim.launch_config()
     if args.launch:
         sim.set_launch_config(**dict(zip(LAUNCH_KEYS, (int(v) for v in args.launch.split(",")))))
         launch = sim.launch_config()
-    elif not args.no_autotune:
+    sim.step(args.warmup)
+    sim.synchronize()
+    if args.autotune and not args.launch:
         t2 = time.perf_counter()
         launch, tune_ms = sim.autotune(1)
         log(f"autotune {time.perf_counter() - t2:.1f}s -> {launch} ({tune_ms * 1e3:.1f} us)")
     t1 = time.perf_counter()
-    total_ms, kernel_ms = sim.time_steps(args.steps)
+    total_ms, kernel_ms, passes, fused = sim.time_passes(args.steps)
     sim.synchronize()
     wall = time.perf_counter() - t1
     ms_per_step = wall * 1e3 / args.steps
     gbs = nv * bpv / (ms_per_step * 1e-3) / 1e9
-    kern_gbs = nv * bpv / (kernel_ms * 1e-3) / 1e9
+    steps_per_launch = args.steps / passes
+    kern_gbs = steps_per_launch * nv * bpv / (kernel_ms * 1e-3) / 1e9
     steps_per_real = cs.host_lib().cs_count_steps(TSTOP_MS, sim.dt)
-    log(f"{args.steps} steps: wall {wall * 1e3:.1f} ms, events {total_ms:.1f} ms, "
-        f"stencil kernel {kernel_ms * 1e3:.1f} us ({kern_gbs:.0f} GB/s)")
+    kernel = "k_step2" if fused == passes else "k_stencil" if fused == 0 else "k_step2+k_stencil"
+    log(f"{args.steps} steps in {passes} passes ({fused} fused): wall {wall * 1e3:.1f} ms, "
+        f"events {total_ms:.1f} ms, pass {kernel_ms * 1e3:.1f} us ({kern_gbs:.0f} GB/s)")
 
     # conservation check of the timed run (total calcium, every slot)
     tot = sim.stats()[2].sum()
@@ -164,8 +173,9 @@ def main():
         "spark_realisations_per_s": 1.0 / (steps_per_real * ms_per_step * 1e-3),
         "roofline": {"bound": "hbm", "achieved": round(kern_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(kern_gbs / PEAK_HBM_GBS, 4), "traffic": traffic,
-                     "kernel": "k_stencil", "kernel_ms": round(kernel_ms, 5),
-                     "algorithmic_bytes_per_launch": nv * bpv, "launch": launch},
+                     "kernel": kernel, "kernel_ms": round(kernel_ms, 5),
+                     "steps_per_launch": round(steps_per_launch, 3),
+                     "algorithmic_bytes_per_launch": round(steps_per_launch * nv * bpv), "launch": launch},
         "cpu_baseline": cpu,
         "total_calcium": tot,
     }

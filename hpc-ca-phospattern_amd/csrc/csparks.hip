@@ -29,6 +29,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -1705,14 +1706,29 @@ int max_fused_zchunk(const Geo& g, size_t esize)
 }
 
 /* Default launch shape, the fastest measured on MI355X for the 32M-voxel
- * single dyad in FP32: linear columns, blocks of 4 wavefronts, 8 planes
- * per block, groups of 16 tiles per XCD, one plane ahead, non-temporal
- * stores of the next state. */
-LaunchCfg default_launch(const Geo& g, size_t esize)
+ * single dyad in FP32.  One-step passes: linear columns, blocks of 4
+ * wavefronts, 8 planes per block, groups of 16 tiles per XCD, one plane
+ * ahead, non-temporal stores of the next state.  Fused passes: groups of
+ * 8 tiles per XCD and the z chunk below. */
+LaunchCfg default_launch(const Geo& g, size_t esize, int cus)
 {
-  LaunchCfg L{4, 8, 16, 1, 1, 0, 1, 0, 0, 50, 1};
+  LaunchCfg L{4, 8, 16, 1, 1, 0, 1, 0, 0, 40, 8};
   L.zchunk = std::min(L.zchunk, max_zchunk(g, esize));
-  L.fz = std::max(1, std::min(L.fz, max_fused_zchunk(g, esize)));
+  /* k_step2: split z into k equal chunks; with one block per CU at a
+   * time, the pass takes ceil(blocks / CUs) rounds of (chunk + 2) planes
+   * (level 1 recomputes one plane on either side) */
+  const int wy = esize == 4 ? 16 : 8;
+  const long tiles = (long)((g.nx + BX - 3) / (BX - 2)) * ((g.ny + wy - 3) / (wy - 2));
+  const int fzmax = max_fused_zchunk(g, esize);
+  double best = 1e30;
+  for (int k = 1; k <= g.nz; k++) {
+    const int zc = (g.nz + k - 1) / k;
+    if (zc > fzmax) continue;
+    const double rounds = std::ceil((double)(tiles * ((g.nz + zc - 1) / zc)) / cus);
+    const double cost = rounds * (zc + 2);
+    if (cost < best - 1e-9) { best = cost; L.fz = zc; }
+  }
+  L.fz = std::max(1, std::min(L.fz, std::max(fzmax, 1)));
   return L;
 }
 
@@ -2419,7 +2435,12 @@ int cs_create(const cs_problem* p, cs_context** out)
   if (hipMalloc(&c->d_partial, sizeof(double) * STAT_BLOCKS * MAXS * MAXD * 3) != hipSuccess ||
       hipMalloc(&c->d_stats, sizeof(double) * MAXS * MAXD * 3) != hipSuccess)
     return bail(set_err(CS_ENOMEM, "device allocation of reduction buffers failed"));
-  c->launch = default_launch(g, c->esize);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) return bail(set_err(CS_EDEVICE, "device query failed"));
+  c->launch = default_launch(g, c->esize, std::max(prop.multiProcessorCount, 1));
+  /* fused passes by default in FP32, where they measured faster; FP64
+   * blocks hold half the rows (register file), so there one-step passes win */
+  c->launch.fuse = (c->f2_ok && c->esize == 4) ? 1 : 0;
   rc = c->esize == 8 ? fill_init<double>(c) : fill_init<float>(c);
   if (rc != CS_OK) return bail(rc);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(set_err(CS_EDEVICE, "setup failed"));

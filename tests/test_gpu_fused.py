@@ -46,6 +46,7 @@ def _run(d, opts, chunks, zchunk=8, xcd=0, oracle=True):
     fu = cs.Simulation(d.geometry, d.model, opts)
     _fused(fu, zchunk, xcd)
     one = cs.Simulation(d.geometry, d.model, opts)
+    one.set_launch_config(fuse=0)
     orc = Oracle(d.geometry, d.model, opts) if oracle else None
     for k in chunks:
         fu.step(k)

@@ -36,10 +36,14 @@ def _compare_fields(sim, orc, rtol):
     return worst
 
 
-def _run_pair(d, opts, nsteps, chunks=1):
-    _say(f"case: {d.description} opts={opts} steps={nsteps}")
+def _run_pair(d, opts, nsteps, chunks=1, fuse=None):
+    """GPU run and oracle over the same steps; fuse 0 / 1 selects one-step
+    or fused two-step passes (None: the library's default)."""
+    _say(f"case: {d.description} opts={opts} steps={nsteps} fuse={fuse}")
     orc = Oracle(d.geometry, d.model, opts)
     sim = cs.Simulation(d.geometry, d.model, opts)
+    if fuse is not None:
+        sim.set_launch_config(fuse=fuse)
     per = nsteps // chunks
     for _ in range(chunks):
         sim.step(per)
@@ -52,43 +56,47 @@ def test_device_visible():
     _say(cs.device_info())
 
 
-def test_fp64_deterministic_small():
+@pytest.mark.parametrize("fuse", [0, 1], ids=["one-step", "fused"])
+def test_fp64_deterministic_small(fuse):
     d = dyad.small_dyad()
     opts = cs.Options(seed=7, open={"ryr": [d.open_index, 4]}, t_close={"ryr": 1.0}, tstop=1.0)
-    sim, orc = _run_pair(d, opts, 300, chunks=3)
+    sim, orc = _run_pair(d, opts, 300, chunks=3, fuse=fuse)
     worst = _compare_fields(sim, orc, RTOL_F64)
     _say(f"fp64 deterministic max rel err {worst:.2e}")
     np.testing.assert_array_equal(sim.channel_states(0), orc.states(0))
     sim.close()
 
 
-def test_fp64_forced_close_and_substeps():
+@pytest.mark.parametrize("fuse", [0, 1], ids=["one-step", "fused"])
+def test_fp64_forced_close_and_substeps(fuse):
     """-C closes the forced-open channel mid-run; -T/-D substeps > 1."""
     d = dyad.small_dyad(seed=5)
     opts = cs.Options(seed=3, open={"ryr": [d.open_index]}, t_close={"ryr": 0.002, "p_ryr": 0.001},
                       dt_update_react=3.5e-5 * 3, dt_update_stoch=3.5e-5 * 2)
-    sim, orc = _run_pair(d, opts, 240, chunks=2)
+    sim, orc = _run_pair(d, opts, 240, chunks=2, fuse=fuse)
     assert sim.problem.reaction_substep > 1 and sim.problem.stochastic_substep > 1
     _compare_fields(sim, orc, RTOL_F64)
     assert sim.channel_states(0).sum() == 0
     sim.close()
 
 
-def test_fp64_refined_geometry():
+@pytest.mark.parametrize("fuse", [0, 1], ids=["one-step", "fused"])
+def test_fp64_refined_geometry(fuse):
     """12 nm geometry run at -h 6: two subdivisions, discrete channels on
     the centre sub-voxel pair (species.c:1003-1020)."""
     d = dyad.small_dyad(n=(10, 9, 7), h=12.0, seed=11)
     opts = cs.Options(h=6.0, seed=1, open={"ryr": [0, 1]}, t_close={"ryr": 1.0})
-    sim, orc = _run_pair(d, opts, 150)
+    sim, orc = _run_pair(d, opts, 150, fuse=fuse)
     assert sim.n == (20, 18, 14)
     _compare_fields(sim, orc, RTOL_F64)
     sim.close()
 
 
-def test_fp64_serca_and_dirichlet():
+@pytest.mark.parametrize("fuse", [0, 1], ids=["one-step", "fused"])
+def test_fp64_serca_and_dirichlet(fuse):
     d = with_serca(dyad.small_dyad(seed=9))
     opts = cs.Options(seed=2, open={"ryr": [d.open_index]}, t_close={"ryr": 1.0}, force_dirichlet=True)
-    sim, orc = _run_pair(d, opts, 200, chunks=2)
+    sim, orc = _run_pair(d, opts, 200, chunks=2, fuse=fuse)
     _compare_fields(sim, orc, RTOL_F64)
     sim.close()
 
@@ -98,7 +106,8 @@ def _with_model(d, **changes):
     return dyad.Dyad(d.geometry, dataclasses.replace(d.model, **changes), d.open_index, d.description)
 
 
-def test_reactions_in_slot_order_when_a_buffer_binds_another_species():
+@pytest.mark.parametrize("fuse", [0, 1], ids=["one-step", "fused"])
+def test_reactions_in_slot_order_when_a_buffer_binds_another_species(fuse):
     """A cleft buffer whose free species is Fluo (not species 0) rules out
     the species-major form: every step takes the reference's slot-order
     path (reaction_SIMD, kernel_generic.cpp:397-509)."""
@@ -107,12 +116,13 @@ def test_reactions_in_slot_order_when_a_buffer_binds_another_species():
     bufs[1][3] = cs.Buffer(4, 1, 0.0327, 0.0196, 70.0)
     d = _with_model(d, buffers=bufs)
     opts = cs.Options(seed=2, open={"ryr": [d.open_index]}, t_close={"ryr": 1.0})
-    sim, orc = _run_pair(d, opts, 120, chunks=2)
+    sim, orc = _run_pair(d, opts, 120, chunks=2, fuse=fuse)
     _compare_fields(sim, orc, RTOL_F64)
     sim.close()
 
 
-def test_reactions_overwrite_when_species0_does_not_diffuse():
+@pytest.mark.parametrize("fuse", [0, 1], ids=["one-step", "fused"])
+def test_reactions_overwrite_when_species0_does_not_diffuse(fuse):
     """Fluo diffuses fastest, so species 0 diffuses on a substep only; on
     the other steps every buffer slot overwrites species 0 with its own
     increment instead of accumulating (reaction_SIMD with update_species[0]
@@ -123,13 +133,14 @@ def test_reactions_overwrite_when_species0_does_not_diffuse():
              for dl, sl in zip(m.diffusive, m.sigma)]
     d = _with_model(d, sigma=sigma)
     opts = cs.Options(seed=2, open={"ryr": [d.open_index]}, t_close={"ryr": 1.0})
-    sim, orc = _run_pair(d, opts, 120, chunks=2)
+    sim, orc = _run_pair(d, opts, 120, chunks=2, fuse=fuse)
     _compare_fields(sim, orc, RTOL_F64)
     sim.close()
 
 
+@pytest.mark.parametrize("fuse", [0, 1], ids=["one-step", "fused"])
 @pytest.mark.parametrize("rng", ["philox", "mt"])
-def test_stochastic_trajectories_match(rng):
+def test_stochastic_trajectories_match(rng, fuse):
     """Same uniforms on both sides => identical channel trajectories."""
     import dataclasses
     d = dyad.small_dyad(seed=4)
@@ -141,7 +152,8 @@ def test_stochastic_trajectories_match(rng):
     opts = cs.Options(seed=123, rng=rng, open={"ryr": [d.open_index]})
     orc = Oracle(d.geometry, model, opts)
     sim = cs.Simulation(d.geometry, model, opts)
-    _say(f"case: stochastic {rng}")
+    sim.set_launch_config(fuse=fuse)
+    _say(f"case: stochastic {rng} fuse={fuse}")
     transitions = 0
     prev = orc.states(0).copy()
     for k in range(40):
@@ -204,7 +216,7 @@ def test_launch_shapes_agree(cfg):
     _say(f"case: launch {cfg}")
     orc = Oracle(d.geometry, d.model, opts)
     sim = cs.Simulation(d.geometry, d.model, opts)
-    sim.set_launch_config(**cfg)
+    sim.set_launch_config(fuse=0, **cfg)   # one-step passes with this shape
     got = sim.launch_config()
     assert {k: got[k] for k in cfg} == {**cfg, "zchunk": min(cfg["zchunk"], sim.n[2])}
     sim.step(60)
@@ -229,7 +241,7 @@ def test_row_kernel_wide_grid(cfg):
     _say(f"case: row kernel {cfg} on {d.description}")
     orc = Oracle(d.geometry, d.model, opts)
     sim = cs.Simulation(d.geometry, d.model, opts)
-    sim.set_launch_config(lds_tile=0, linear=0, **cfg)
+    sim.set_launch_config(lds_tile=0, linear=0, fuse=0, **cfg)
     sim.step(40)
     orc.step(40)
     _compare_fields(sim, orc, RTOL_F64)
