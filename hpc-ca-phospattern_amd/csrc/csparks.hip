@@ -182,6 +182,18 @@ __device__ inline double dpp_from_right(double v, double edge)
   return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
+/* The same moves with 0 for the lane that has no source (bound_ctrl). */
+__device__ inline float dpp_from_left_zero(float v)
+{
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, true));
+}
+__device__ inline float dpp_from_right_zero(float v)
+{
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, true));
+}
+__device__ inline double dpp_from_left_zero(double v) { return dpp_from_left(v, 0.0); }
+__device__ inline double dpp_from_right_zero(double v) { return dpp_from_right(v, 0.0); }
+
 /* Tile handled by hardware block b.  Blocks are dispatched to the 8 XCDs
  * round-robin (b % 8).  xcd 0 keeps that order; xcd 1 gives XCD k the
  * contiguous tiles [k*per, (k+1)*per); xcd = G >= 2 hands out groups of G
@@ -1232,11 +1244,19 @@ k_step2(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__
 #pragma unroll
     for (int s = 0; s < NS; s++)
       if ((dmask >> s) & 1u) {
-        nxm[s] = dpp_from_left(vc[s], xe[s]);
-        nxp[s] = dpp_from_right(vc[s], xe[s]);
-        if (!first && xclamp) {
-          nxm[s] = has_xm ? nxm[s] : vc[s];
-          nxp[s] = has_xp ? nxp[s] : vc[s];
+        if (first) {
+          nxm[s] = dpp_from_left(vc[s], xe[s]);
+          nxp[s] = dpp_from_right(vc[s], xe[s]);
+        } else {
+          /* level 2: the first and last lane are never inner, so their
+           * missing neighbour may read 0, which lets the move fold into
+           * the difference that uses it */
+          nxm[s] = dpp_from_left_zero(vc[s]);
+          nxp[s] = dpp_from_right_zero(vc[s]);
+          if (xclamp) {
+            nxm[s] = has_xm ? nxm[s] : vc[s];
+            nxp[s] = has_xp ? nxp[s] : vc[s];
+          }
         }
       }
     if (__ballot(!uni) == 0) {
