@@ -1418,6 +1418,30 @@ k_step2(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__
   }
 }
 
+/* Flat records of the boundary fluxes for k_bridge, resolved on the host
+ * so that a channel, a row or an entry is one load: element indices of
+ * the field (species s at voxel v), level-1 slots, resting values. */
+struct BChan {          /* a channel of a discrete flux, in gating order */
+  uint64_t elem_to;     /* species 0 at the channel's cleft (to) voxel    */
+  double rest_to;       /* resting species 0 there                        */
+  uint32_t slot_to;     /* level-1 slot of that voxel                     */
+  uint32_t flux, ch;    /* flux index, channel index within the flux      */
+  uint32_t pad;
+};
+struct BRow {           /* a k_flux row                                   */
+  uint64_t elem;        /* the value the row adds to                      */
+  uint32_t start, end;  /* its entries                                    */
+  uint32_t skip, pad;   /* 1: species fixed in the domain, or species 0 on
+                           a Dirichlet face (the row takes nothing)       */
+};
+struct BEnt {           /* a pair of a row, in k_flux's order             */
+  uint64_t elem_from, elem_to;
+  double rest_from, rest_to;
+  uint32_t slot_from, slot_to;
+  uint32_t state;       /* the channel's state index, or NO_ROW           */
+  uint32_t flux_sign;   /* flux | (row at the to voxel) << 8              */
+};
+
 /* Channel gating and boundary fluxes around fused passes, in one block
  * with barriers between the phases:
  *   tail, after a pass over steps t, t+1: k_gate and k_flux of step t+1,
@@ -1426,42 +1450,78 @@ k_step2(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__
  *   head, before a pass over steps t', t'+1: k_gate of step t' on u, then
  *     the flux sum of every k_flux row from u, which the pass adds at
  *     level 1.
- * Either part may be off.  The same gate_channels / row_sum code as
- * k_gate / k_flux gives the same results. */
+ * Either part may be off.  The gating (gate_one) and flux (flux_j) code,
+ * the entry order and the skip rules are k_gate's and k_flux's, so the
+ * results are the same. */
 template <typename T>
 __global__ void __launch_bounds__(BRIDGE_THREADS)
-k_bridge(T* __restrict__ u, const uint8_t* __restrict__ ids, const DevParams<T>* __restrict__ P, Geo g,
-         const uint32_t* __restrict__ pairs, const uint32_t* __restrict__ pslot, const T* __restrict__ l1,
-         uint8_t* __restrict__ states, uint8_t* __restrict__ recruited, FluxRows rows, double* __restrict__ fsum,
+k_bridge(T* __restrict__ u, const DevParams<T>* __restrict__ P, const BChan* __restrict__ chans, uint32_t nchan,
+         const BRow* __restrict__ rows, uint32_t nrows, const BEnt* __restrict__ ents, const T* __restrict__ l1,
+         uint8_t* __restrict__ states, uint8_t* __restrict__ recruited, double* __restrict__ fsum,
          uint8_t* __restrict__ fany, F2Step tail, uint32_t do_tail, F2Step head, uint32_t do_head, uint32_t D,
          uint32_t seed, uint32_t instance, const double* __restrict__ uniforms)
 {
+  __shared__ FluxParams F[MAXF];
+  {
+    const uint32_t words = (uint32_t)(sizeof(FluxParams) * MAXF / sizeof(uint64_t));
+    const uint64_t* src = reinterpret_cast<const uint64_t*>(P->flux);
+    uint64_t* dst = reinterpret_cast<uint64_t*>(F);
+    for (uint32_t k = threadIdx.x; k < words; k += blockDim.x) dst[k] = src[k];
+  }
+  __syncthreads();
+  const double dt = P->dt, h = P->h, dts = dt * D;
+
+  auto gate = [&](const F2Step& st, auto ca) {
+    const double t = (double)st.ti * dt;
+    for (uint32_t c = threadIdx.x; c < nchan; c += blockDim.x) {
+      const BChan b = chans[c];
+      const FluxParams& Fp = F[b.flux];
+      const uint32_t k = Fp.state_off + b.ch;
+      uint8_t sv = states[k];
+      const double c0 = Fp.stochastic ? ca(b) + b.rest_to : 0.0;
+      sv = gate_one(Fp, sv, c0, st.ti, t, dts, seed, instance, uniforms, st.uni_base, b.ch);
+      states[k] = sv;
+      if (sv) recruited[k] = 1;
+    }
+  };
+  auto row_flux = [&](uint32_t r, uint32_t fluxmask, auto val, double& sum) -> bool {
+    const BRow w = rows[r];
+    sum = 0.0;
+    if (w.skip) return false;
+    bool any = false;
+    for (uint32_t e = w.start; e < w.end; e++) {
+      const BEnt x = ents[e];
+      const uint32_t f = x.flux_sign & 0xFFu;
+      if (!((fluxmask >> f) & 1u)) continue;
+      if (x.state != NO_ROW && !states[x.state]) continue;
+      const double J = flux_j(F[f], val(x, false), val(x, true), x.rest_from, x.rest_to, dt, h);
+      sum += (x.flux_sign >> 8) ? J : -J;
+      any = true;
+    }
+    return any;
+  };
+
   if (do_tail) {
-    if (tail.gate)
-      gate_channels<T>(P, ids, pairs, states, recruited, tail.ti, D, seed, instance, uniforms, tail.uni_base,
-                       [&](uint32_t, uint32_t gp) { return (double)l1[pslot[2 * gp + 1]]; });
+    if (tail.gate) gate(tail, [&](const BChan& b) { return (double)l1[b.slot_to]; });
     __syncthreads();
     if (tail.fluxmask)
-      for (uint32_t r = threadIdx.x; r < rows.n; r += blockDim.x) {
-        double sum = 0.0;
-        const uint32_t v = rows.voxel[r];
-        if (row_sum<T>(P, ids, g, rows, r, pairs, states, tail.fluxmask, sum,
-                       [&](uint32_t, uint32_t, uint32_t gp, uint32_t which) { return (double)l1[pslot[2 * gp + which]]; })) {
-          T* o = u + elem(g, 0, v);
+      for (uint32_t r = threadIdx.x; r < nrows; r += blockDim.x) {
+        double sum;
+        if (row_flux(r, tail.fluxmask,
+                     [&](const BEnt& x, bool to) { return (double)l1[to ? x.slot_to : x.slot_from]; }, sum)) {
+          T* o = u + rows[r].elem;
           *o = (T)((double)*o + sum);
         }
       }
     __syncthreads();
   }
   if (do_head) {
-    if (head.gate)
-      gate_channels<T>(P, ids, pairs, states, recruited, head.ti, D, seed, instance, uniforms, head.uni_base,
-                       [&](uint32_t to, uint32_t) { return (double)u[elem(g, 0, to)]; });
+    if (head.gate) gate(head, [&](const BChan& b) { return (double)u[b.elem_to]; });
     __syncthreads();
-    for (uint32_t r = threadIdx.x; r < rows.n; r += blockDim.x) {
-      double sum = 0.0;
-      const bool any = row_sum<T>(P, ids, g, rows, r, pairs, states, head.fluxmask, sum,
-                                  [&](uint32_t s, uint32_t vox, uint32_t, uint32_t) { return (double)u[elem(g, s, vox)]; });
+    for (uint32_t r = threadIdx.x; r < nrows; r += blockDim.x) {
+      double sum;
+      const bool any = row_flux(r, head.fluxmask,
+                                [&](const BEnt& x, bool to) { return (double)u[to ? x.elem_to : x.elem_from]; }, sum);
       fsum[r] = sum;
       fany[r] = any ? 1 : 0;
     }
@@ -1692,6 +1752,10 @@ struct cs_context {
   void* d_f2_l1 = nullptr;           /* level-1 species 0 at the pair voxels  */
   double* d_f2_fsum = nullptr;       /* step-1 flux sum per k_flux row        */
   uint8_t* d_f2_fany = nullptr;      /* the row takes its sum                 */
+  void* d_bchan = nullptr;           /* k_bridge records: BChan, BRow, BEnt   */
+  void* d_brow = nullptr;
+  void* d_bent = nullptr;
+  uint32_t nbchan = 0;
   bool f2_ok = false;                /* the problem admits fused passes       */
   std::string f2_why;                /* why not, when it does not             */
   uint32_t fsm = 0;                  /* species the fluxes act on (bit 0)     */
@@ -1760,7 +1824,8 @@ void free_ctx(cs_context* c)
                   c->d_row_voxel, c->d_row_species, c->d_row_start, c->d_ent_pair,
                   c->d_ent_flux, c->d_ent_sign, c->d_uniforms, c->d_partial, c->d_stats,
                   c->d_lweights, c->d_lpart, c->d_f2_col, c->d_f2_z,
-                  c->d_f2_row, c->d_f2_pslot, c->d_f2_l1, c->d_f2_fsum, c->d_f2_fany};
+                  c->d_f2_row, c->d_f2_pslot, c->d_f2_l1, c->d_f2_fsum, c->d_f2_fany, c->d_bchan,
+                  c->d_brow, c->d_bent};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -2142,11 +2207,11 @@ template <typename T>
 void launch_bridge(cs_context* c, void* u, const F2Step& tail, bool do_tail, const F2Step& head, bool do_head,
                    const double* d_uni)
 {
-  hipLaunchKernelGGL(k_bridge<T>, dim3(1), dim3(BRIDGE_THREADS), 0, c->stream, (T*)u, c->d_ids,
-                     (const DevParams<T>*)c->d_params, c->geo, c->d_pairs, c->d_f2_pslot, (const T*)c->d_f2_l1,
-                     c->d_states, c->d_recruited, flux_rows(c), c->d_f2_fsum, c->d_f2_fany, tail,
-                     do_tail ? 1u : 0u, head, do_head ? 1u : 0u, (uint32_t)c->prob.stochastic_substep,
-                     (uint32_t)c->prob.seed, (uint32_t)c->prob.instance, d_uni);
+  hipLaunchKernelGGL(k_bridge<T>, dim3(1), dim3(BRIDGE_THREADS), 0, c->stream, (T*)u,
+                     (const DevParams<T>*)c->d_params, (const BChan*)c->d_bchan, c->nbchan,
+                     (const BRow*)c->d_brow, c->nrows, (const BEnt*)c->d_bent, (const T*)c->d_f2_l1, c->d_states,
+                     c->d_recruited, c->d_f2_fsum, c->d_f2_fany, tail, do_tail ? 1u : 0u, head, do_head ? 1u : 0u,
+                     (uint32_t)c->prob.stochastic_substep, (uint32_t)c->prob.seed, (uint32_t)c->prob.instance, d_uni);
 }
 
 uint64_t gates_in(const cs_context* c, uint64_t first, uint64_t n)
@@ -2365,6 +2430,46 @@ int build_flux_rows(cs_context* c, const cs_problem* p)
   if ((rc = up(&c->d_f2_z, vz)) != CS_OK) return rc;
   if ((rc = up(&c->d_f2_row, vrow)) != CS_OK) return rc;
   if ((rc = up(&c->d_f2_pslot, pslot)) != CS_OK) return rc;
+
+  /* k_bridge's flat records */
+  std::vector<uint32_t> pair_off(p->num_fluxes + 1, 0);
+  for (uint32_t f = 0; f < p->num_fluxes; f++) pair_off[f + 1] = pair_off[f] + p->fluxes[f].num_pairs;
+  auto dom = [c](uint32_t v) { return (int)(c->h_ids[v] & 7); };
+  std::vector<BChan> bchan;
+  for (uint32_t f = 0; f < p->num_fluxes; f++) {
+    if (!p->fluxes[f].discrete) continue;
+    for (uint32_t ch = 0; ch < p->fluxes[f].num_pairs; ch++) {
+      const uint32_t gp = pair_off[f] + ch, to = pairs_dev[2 * gp + 1];
+      bchan.push_back(BChan{elem(g, 0, to), p->init[dom(to)][0], pslot[2 * gp + 1], f, ch, 0});
+    }
+  }
+  std::vector<BRow> brow(row_voxel.size());
+  std::vector<BEnt> bent(ent_pair.size());
+  for (size_t r = 0; r < row_voxel.size(); r++) {
+    const uint32_t v = row_voxel[r], sp = row_species[r];
+    const uint32_t x = v % (uint32_t)g.nx, y = (v / (uint32_t)g.nx) % (uint32_t)g.ny, z = (uint32_t)(v / g.plane);
+    const bool face = x == 0 || y == 0 || z == 0 || x + 1 == (uint32_t)g.nx || y + 1 == (uint32_t)g.ny ||
+                      z + 1 == (uint32_t)g.nz;
+    const bool skip = p->fixed[dom(v)][sp] || (sp == 0 && p->force_dirichlet && face);
+    brow[r] = BRow{elem(g, sp, v), row_start[r], row_start[r + 1], skip ? 1u : 0u, 0};
+    for (uint32_t e = row_start[r]; e < row_start[r + 1]; e++) {
+      const uint32_t f = ent_flux[e], gp = pair_off[f] + ent_pair[e];
+      const uint32_t from = pairs_dev[2 * gp], to = pairs_dev[2 * gp + 1];
+      bent[e] = BEnt{elem(g, sp, from), elem(g, sp, to), p->init[dom(from)][sp], p->init[dom(to)][sp],
+                     pslot[2 * gp], pslot[2 * gp + 1],
+                     p->fluxes[f].discrete ? c->state_off[f] + ent_pair[e] : NO_ROW,
+                     f | (ent_sign[e] > 0 ? 1u : 0u) << 8};
+    }
+  }
+  c->nbchan = (uint32_t)bchan.size();
+  auto upb = [c](void** dptr, const void* src, size_t bytes) -> int {
+    HIP_TRY(hipMalloc(dptr, std::max<size_t>(bytes, 16)));
+    if (bytes) HIP_TRY(hipMemcpyAsync(*dptr, src, bytes, hipMemcpyHostToDevice, c->stream));
+    return CS_OK;
+  };
+  if ((rc = upb(&c->d_bchan, bchan.data(), sizeof(BChan) * bchan.size())) != CS_OK) return rc;
+  if ((rc = upb(&c->d_brow, brow.data(), sizeof(BRow) * brow.size())) != CS_OK) return rc;
+  if ((rc = upb(&c->d_bent, bent.data(), sizeof(BEnt) * bent.size())) != CS_OK) return rc;
   HIP_TRY(hipMalloc(&c->d_f2_l1, c->esize * std::max<size_t>(keys.size(), 1)));
   HIP_TRY(hipMalloc((void**)&c->d_f2_fsum, sizeof(double) * std::max<size_t>(row_voxel.size(), 1)));
   HIP_TRY(hipMalloc((void**)&c->d_f2_fany, std::max<size_t>(row_voxel.size(), 1)));
