@@ -51,7 +51,7 @@ constexpr int FLUX_THREADS = 256;
 constexpr int BRIDGE_THREADS = 1024;
 constexpr size_t F2_MAX_ROWS = 16384;   /* rows and channels k_bridge's one block takes */
 constexpr int STAT_THREADS = 256;
-constexpr int STAT_BLOCKS = 1024;
+constexpr int STAT_BLOCKS = 4096;
 
 thread_local std::string g_err;
 
@@ -1551,10 +1551,21 @@ __device__ inline void block_reduce3(double (&mn)[MAXD], double (&mx)[MAXD], dou
   }
 }
 
+/* Voxels per domain, passed by value to k_stats_final. */
+struct DomCounts {
+  uint64_t n[MAXD];
+};
+
+/* Block partials of the output reductions, per (species, domain): the
+ * minimum and maximum of the stored deviations and their sum times dV.
+ * The resting value is added once per domain by k_stats_final: rounding
+ * is monotone, so min(u' + r) = min(u') + r exactly, and the sum is
+ * sum(u') dV + n r dV.  A wavefront whose lanes share one domain updates
+ * only that domain's accumulators. */
 template <typename T>
 __global__ void __launch_bounds__(STAT_THREADS)
-k_stats(const T* __restrict__ u, const uint8_t* __restrict__ ids, const DevParams<T>* __restrict__ P, Geo g,
-        uint32_t ns, double dV, double* __restrict__ partial /* [grid][ns][MAXD][3] */)
+k_stats(const T* __restrict__ u, const uint8_t* __restrict__ ids, Geo g, uint32_t ns, double dV,
+        double* __restrict__ partial /* [grid][ns][MAXD][3] */)
 {
   /* block (bx, by) takes planes by, by + gridDim.y, ... and within each the
    * columns bx*blockDim + t strided by the grid width: a fixed assignment,
@@ -1562,27 +1573,65 @@ k_stats(const T* __restrict__ u, const uint8_t* __restrict__ ids, const DevParam
   __shared__ double sh[3 * MAXD * (STAT_THREADS / 64)];
   const int nw = STAT_THREADS / 64;
   const uint32_t blk = blockIdx.y * gridDim.x + blockIdx.x;
+  const T inf = (T)INFINITY;
   for (uint32_t s = 0; s < ns; s++) {
-    double mn[MAXD], mx[MAXD], sm[MAXD];
+    T mn[MAXD], mx[MAXD];
+    double sm[MAXD];
 #pragma unroll
-    for (int d = 0; d < MAXD; d++) { mn[d] = FLT_MAX; mx[d] = FLT_MIN; sm[d] = 0.0; }
+    for (int d = 0; d < MAXD; d++) { mn[d] = inf; mx[d] = -inf; sm[d] = 0.0; }
+    auto upd = [&](int d, T v) {
+      mn[d] = v < mn[d] ? v : mn[d];
+      mx[d] = v > mx[d] ? v : mx[d];
+      sm[d] += (double)v;
+    };
+    /* eight strided elements per lane are loaded before any is used, so
+     * each wavefront keeps several loads in flight */
+    constexpr int U = 8;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (int z = blockIdx.y; z < g.nz; z += gridDim.y) {
       const T* us = u + ((uint64_t)z * ns + s) * g.plane;
       const uint8_t* iz = ids + (uint64_t)z * g.plane;
-      for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < g.plane;
-           c += (uint64_t)gridDim.x * blockDim.x) {
-        const int dv = iz[c] & 7;
-        const double val = (double)us[c] + P->init64[dv][s];   /* deviation + resting value */
+      for (uint64_t c0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c0 < g.plane; c0 += U * stride) {
+        T v[U];
+        int dv[U];
 #pragma unroll
-        for (int d = 0; d < MAXD; d++)
-          if (d == dv) {
-            mn[d] = val < mn[d] ? val : mn[d];
-            mx[d] = val > mx[d] ? val : mx[d];
-            sm[d] += val * dV;
+        for (int k = 0; k < U; k++) {
+          const uint64_t c = c0 + k * stride;
+          dv[k] = -1;   /* past the plane: no domain */
+          v[k] = T(0);
+          if (c < g.plane) {
+            dv[k] = iz[c] & 7;
+            v[k] = us[c];
           }
+        }
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+          const int d0 = __builtin_amdgcn_readfirstlane(dv[k]);
+          if (__ballot(dv[k] != d0) == 0) {
+            switch (d0) {
+              case 0: upd(0, v[k]); break;
+              case 1: upd(1, v[k]); break;
+              case 2: upd(2, v[k]); break;
+              case 3: upd(3, v[k]); break;
+              case 4: upd(4, v[k]); break;
+              default: break;   /* every lane past the plane */
+            }
+          } else {
+#pragma unroll
+            for (int d = 0; d < MAXD; d++)
+              if (d == dv[k]) upd(d, v[k]);
+          }
+        }
       }
     }
-    block_reduce3<T>(mn, mx, sm, sh);
+    double dmn[MAXD], dmx[MAXD];
+#pragma unroll
+    for (int d = 0; d < MAXD; d++) {
+      dmn[d] = (double)mn[d];
+      dmx[d] = (double)mx[d];
+      sm[d] *= dV;
+    }
+    block_reduce3<T>(dmn, dmx, sm, sh);
     __syncthreads();
     if (threadIdx.x < MAXD) {
       const int d = threadIdx.x;
@@ -1600,14 +1649,19 @@ k_stats(const T* __restrict__ u, const uint8_t* __restrict__ ids, const DevParam
 }
 
 /* One block per (species, domain) entry: threads stride over the block
- * partials, then a fixed-order tree in LDS combines them. */
+ * partials, a fixed-order tree in LDS combines them, and the resting
+ * value r joins: min = min(FLT_MAX, min u' + r), max = max(FLT_MIN,
+ * max u' + r) -- the reference's start values (species.c:2821-2953) --
+ * and sum = sum u' dV + n r dV. */
+template <typename T>
 __global__ void __launch_bounds__(STAT_THREADS)
 k_stats_final(const double* __restrict__ partial, uint32_t nblocks, uint32_t ns,
+              const DevParams<T>* __restrict__ P, DomCounts cnt, double dV,
               double* __restrict__ out /* [ns][MAXD][3] */)
 {
   __shared__ double sa[STAT_THREADS], sb[STAT_THREADS], sc[STAT_THREADS];
   const uint32_t i = blockIdx.x;   /* s * MAXD + d */
-  double a = FLT_MAX, b = FLT_MIN, c = 0.0;
+  double a = INFINITY, b = -INFINITY, c = 0.0;
   for (uint32_t k = threadIdx.x; k < nblocks; k += blockDim.x) {
     const double* p = partial + ((uint64_t)k * ns * MAXD + i) * 3;
     a = p[0] < a ? p[0] : a;
@@ -1624,7 +1678,13 @@ k_stats_final(const double* __restrict__ partial, uint32_t nblocks, uint32_t ns,
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) { out[3 * i + 0] = sa[0]; out[3 * i + 1] = sb[0]; out[3 * i + 2] = sc[0]; }
+  if (threadIdx.x == 0) {
+    const uint32_t s = i / MAXD, d = i % MAXD;
+    const double r = P->init64[d][s];
+    out[3 * i + 0] = fmin((double)FLT_MAX, sa[0] + r);
+    out[3 * i + 1] = fmax((double)FLT_MIN, sb[0] + r);
+    out[3 * i + 2] = sc[0] + (double)cnt.n[d] * r * dV;
+  }
 }
 
 /* Linescan reductions (species.c:1960-2035).  The reference's 3-D
@@ -1744,6 +1804,7 @@ struct cs_context {
   LaunchCfg launch{4, 32, 1, 1, 0, 0, 0, 0, 0, 32, 1};
   uint32_t rfast = 0;   /* buffer reactions admit the species-major form */
   std::vector<uint8_t> h_ids;   /* domain ids in device order (host copy)  */
+  uint64_t dcount[MAXD] = {0};  /* voxels per domain                        */
   /* fused two-step pass (k_step2) */
   uint32_t* d_f2_col = nullptr;      /* pair voxels by column, see F2Voxels   */
   uint32_t* d_f2_z = nullptr;
@@ -2553,6 +2614,8 @@ int cs_create(const cs_problem* p, cs_context** out)
       hipStreamSynchronize(c->stream) != hipSuccess)
     return bail(set_err(CS_EDEVICE, "upload of ids failed"));
   c->h_ids.swap(ids);
+  for (uint8_t id : c->h_ids)
+    if ((id & 7) < MAXD) c->dcount[id & 7]++;
   rc = build_flux_rows(c, p);
   if (rc != CS_OK) return bail(rc);
   rc = c->esize == 8 ? upload_params<double>(c) : upload_params<float>(c);
@@ -2713,16 +2776,19 @@ int cs_reduce_stats(cs_context* c, double* mn, double* mx, double* tot)
   const uint32_t gy = (uint32_t)std::min<int>(STAT_BLOCKS / 16, c->geo.nz);
   const uint32_t blocks = gx * gy;
   const double dV = c->prob.dV;
-  if (c->esize == 8)
+  DomCounts cnt{};
+  for (uint32_t d = 0; d < MAXD; d++) cnt.n[d] = c->dcount[d];
+  if (c->esize == 8) {
     hipLaunchKernelGGL(k_stats<double>, dim3(gx, gy), dim3(STAT_THREADS), 0, c->stream,
-                       (const double*)c->d_u[c->cur], c->d_ids, (const DevParams<double>*)c->d_params, c->geo, c->ns,
-                       dV, c->d_partial);
-  else
+                       (const double*)c->d_u[c->cur], c->d_ids, c->geo, c->ns, dV, c->d_partial);
+    hipLaunchKernelGGL(k_stats_final<double>, dim3(c->ns * MAXD), dim3(STAT_THREADS), 0, c->stream, c->d_partial,
+                       blocks, c->ns, (const DevParams<double>*)c->d_params, cnt, dV, c->d_stats);
+  } else {
     hipLaunchKernelGGL(k_stats<float>, dim3(gx, gy), dim3(STAT_THREADS), 0, c->stream,
-                       (const float*)c->d_u[c->cur], c->d_ids, (const DevParams<float>*)c->d_params, c->geo, c->ns,
-                       dV, c->d_partial);
-  hipLaunchKernelGGL(k_stats_final, dim3(c->ns * MAXD), dim3(STAT_THREADS), 0, c->stream, c->d_partial, blocks,
-                     c->ns, c->d_stats);
+                       (const float*)c->d_u[c->cur], c->d_ids, c->geo, c->ns, dV, c->d_partial);
+    hipLaunchKernelGGL(k_stats_final<float>, dim3(c->ns * MAXD), dim3(STAT_THREADS), 0, c->stream, c->d_partial,
+                       blocks, c->ns, (const DevParams<float>*)c->d_params, cnt, dV, c->d_stats);
+  }
   HIP_TRY(hipGetLastError());
   double host[MAXS * MAXD * 3];
   HIP_TRY(hipMemcpyAsync(host, c->d_stats, sizeof(double) * c->ns * MAXD * 3, hipMemcpyDeviceToHost, c->stream));
