@@ -1384,10 +1384,12 @@ k_step2(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__
     }
     const int zo = z - 1;
     if (zo >= z0) {
+      /* y neighbours of level 1; only the species level 2 diffuses are
+       * read, so the others are left as they are */
       T Lm[NS], Lp[NS], nw[NS];
 #pragma unroll
       for (int s = 0; s < NS; s++) {
-        Lm[s] = Lp[s] = wc[s];
+        Lm[s] = Lp[s] = T(0);
         if ((s2.dmask >> s) & 1u) {
           Lm[s] = ex[zo & 1][s][wym][lane];
           Lp[s] = ex[zo & 1][s][wyp][lane];
@@ -1853,8 +1855,8 @@ int max_fused_zchunk(const Geo& g, size_t esize)
 /* Default launch shape, the fastest measured on MI355X for the 32M-voxel
  * single dyad in FP32.  One-step passes: linear columns, blocks of 4
  * wavefronts, 8 planes per block, groups of 16 tiles per XCD, one plane
- * ahead, non-temporal stores of the next state.  Fused passes: groups of
- * 8 tiles per XCD and the z chunk below. */
+ * ahead, non-temporal stores of the next state.  Fused passes: the z
+ * chunk and XCD groups below. */
 LaunchCfg default_launch(const Geo& g, size_t esize, int cus)
 {
   LaunchCfg L{4, 8, 16, 1, 1, 0, 1, 0, 0, 40, 8};
@@ -1874,6 +1876,13 @@ LaunchCfg default_launch(const Geo& g, size_t esize, int cus)
     if (cost < best - 1e-9) { best = cost; L.fz = zc; }
   }
   L.fz = std::max(1, std::min(L.fz, std::max(fzmax, 1)));
+  /* XCD groups of up to 8 consecutive tiles (measured best; larger
+   * groups were within noise), small enough that every XCD gets at least
+   * four groups (hardware order for small grids) */
+  const long nb = tiles * ((g.nz + L.fz - 1) / L.fz);
+  L.fx = 0;
+  for (int G = 8; G >= 2; G /= 2)
+    if (nb >= 8L * 4 * G) { L.fx = G; break; }
   return L;
 }
 
