@@ -1,17 +1,19 @@
 /*
  * csparks.hip -- MI355X (gfx950) time stepping for calcium_sparks.
  *
- * Device layout.  Every species is one array in HBM indexed
- * (z*ny + y)*nx + x, x fastest, where (x, y, z) are the reference's axes
- * 0, 1, 2; the species arrays are stacked [species][z][y][x].  Two such
- * stacks ping-pong: a step reads stack `cur` and writes the complete next
- * state into the other, which replaces the reference's du buffer and its
- * apply_du copy (species.c:1080-1121).  Domain ids are one byte per voxel:
- * bits 0-2 hold the domain, bit 7 is set when all six in-grid neighbours
- * share that domain (the reference's stencil_test, kernel_generic.cpp:
- * 125-173, inverted).
+ * Device layout.  Fields are stored [z][species][y][x], x fastest, where
+ * (x, y, z) are the reference's axes 0, 1, 2: the ns species planes of one
+ * z lie next to each other.  Each value is the deviation of the field from
+ * its voxel's resting value init[domain][species], which keeps FP32 fields
+ * accurate next to large resting values.  Two such stacks ping-pong: a
+ * pass reads stack `cur` and writes the complete next state into the
+ * other, which replaces the reference's du buffer and its apply_du copy
+ * (species.c:1080-1121).  Domain ids are one byte per voxel: bits 0-2 hold
+ * the domain, bit 7 is set when all six in-grid neighbours share that
+ * domain (the reference's stencil_test, kernel_generic.cpp:125-173,
+ * inverted).
  *
- * One step = three launches on the context stream:
+ * A one-step pass is three launches on the context stream:
  *   k_gate    (every stochastic substep) one block updates the channel
  *             states from Ca at the channel mouths (evaluate_RyR_
  *             stochastically, boundaryfluxes.c:295-359);
@@ -24,6 +26,11 @@
  *   k_flux    one thread per membrane voxel gathers every boundary-pair
  *             flux touching it (species.c:1704-1807) -- a gather, so no
  *             floating-point atomics and a fixed summation order.
+ * A fused pass computes two steps per trip through HBM:
+ *   k_step2   both steps' stencils and reactions on tiles with a one-voxel
+ *             halo, the intermediate state kept on chip;
+ *   k_bridge  one block gates the channels and applies the boundary
+ *             fluxes around the passes, with k_gate's and k_flux's code.
  */
 #include <hip/hip_runtime.h>
 
@@ -318,33 +325,6 @@ __device__ inline void store_next(T* p, T v, bool nt)
 {
   if (nt) __builtin_nontemporal_store(v, p);
   else *p = v;
-}
-
-/* A wavefront-uniform pointer, passed through readfirstlane so that loop
- * optimisations cannot fold it into per-lane 64-bit address arithmetic. */
-/* Loads and stores at a wavefront-uniform base plus a per-lane 32-bit
- * byte offset (the global_load/store saddr form: no per-lane address
- * arithmetic).  Inside loops the offsets go through pin_offsets() first:
- * otherwise the compiler hoists their zero extension out of the loop and
- * falls back to 64-bit per-lane addresses. */
-template <typename T>
-__device__ inline T load_at(const T* base, uint32_t byte_off)
-{
-  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
-}
-
-template <typename T>
-__device__ inline void store_at(T* base, uint32_t byte_off, T v, bool nt)
-{
-  store_next(reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off), v, nt);
-}
-
-__device__ inline void pin_offset(uint32_t& v) { asm volatile("" : "+v"(v)); }
-
-template <typename... U>
-__device__ inline void pin_offsets(U&... v)
-{
-  (pin_offset(v), ...);
 }
 
 /* Raw buffer access: a descriptor (base and byte size, uniform across the
