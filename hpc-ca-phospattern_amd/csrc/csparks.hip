@@ -3085,6 +3085,8 @@ int cs_autotune(cs_context* c, uint64_t first_time_ind, uint32_t reps, cs_launch
     for (int zc : {16, 25, 32, 50, 64, 100})
       if (zc < nz && zc <= fzmax) fzs.push_back(zc);
     if (nz <= fzmax) fzs.push_back(nz);
+    if (std::find(fzs.begin(), fzs.end(), c->launch.fz) == fzs.end() && c->launch.fz <= fzmax)
+      fzs.push_back(c->launch.fz);   /* the default's balanced chunk */
     double fms = 1e30;
     LaunchCfg fbest = winner;
     for (int zc : fzs)
