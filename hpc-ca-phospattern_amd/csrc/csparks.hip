@@ -1194,7 +1194,15 @@ k_step2(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__
   auto iofs = [&](int zz) { return (uint32_t)(min(max(zz, zb), ze - 1) - zb) * (uint32_t)plane; };
 
   const uint32_t react_nb = (uint32_t)P->nb;
-  const uint32_t rbound = P->rbound;
+  uint32_t dm1 = s1.dmask, dm2 = s2.dmask, re1 = s1.react, re2 = s2.react, rbound = P->rbound;
+  __shared__ uint32_t msk[5];
+  if (lane == 0 && wy == 0) {
+    msk[0] = dm1;
+    msk[1] = dm2;
+    msk[2] = re1;
+    msk[3] = re2;
+    msk[4] = rbound;
+  }
   const T dtr = P->dt_react;
   const bool dirichlet = P->dirichlet != 0;
 
@@ -1300,35 +1308,36 @@ k_step2(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__
   };
 
   /* register rings, rotated by renaming over their common period of six
-   * planes: level 0 holds planes z-1 .. z+3 and receives z+4; level 1
-   * holds planes z-2, z-1 and receives z.  The y (and edge-lane x)
-   * neighbours of level 0 at plane z are reloaded for z+1 as soon as
-   * level 1 has used them.  Values of species a level does not diffuse
-   * are never read from the neighbour registers. */
+   * planes: level 0 holds planes z-1 .. z+2 and receives z+3 (one slot is
+   * free); level 1 holds planes z-2, z-1 and receives z.  The y (and
+   * edge-lane x) neighbours of level 0 at plane z are reloaded for z+1 as
+   * soon as level 1 has used them.  A lane without an outer x neighbour
+   * in the grid loads its own column, the mirror of the face.  Values of
+   * species a level does not diffuse are never read from the neighbour
+   * registers. */
+  const uint32_t bxe = edge_lane ? bedge : bc;
   T A0[NS], A1[NS], A2[NS], A3[NS], A4[NS], A5[NS];
   T Ym[NS], Yp[NS], Xe[NS];
   T B0[NS], B1[NS], B2[NS];
-  uint32_t I0, I1, I2, I3, I4, I5 = 0;
+  uint32_t I0, I1, I2, I3, I4 = 0, I5 = 0;
 #pragma unroll
   for (int s = 0; s < NS; s++) {
     A0[s] = bload<T>(rin, bc, gofs(zs - 1) + s * pbytes);
     A1[s] = bload<T>(rin, bc, gofs(zs) + s * pbytes);
     A2[s] = bload<T>(rin, bc, gofs(zs + 1) + s * pbytes);
     A3[s] = bload<T>(rin, bc, gofs(zs + 2) + s * pbytes);
-    A4[s] = bload<T>(rin, bc, gofs(zs + 3) + s * pbytes);
-    A5[s] = A1[s];
+    A4[s] = A5[s] = A1[s];
     Ym[s] = Yp[s] = Xe[s] = A1[s];
-    if ((s1.dmask >> s) & 1u) {
+    if ((dm1 >> s) & 1u) {
       Ym[s] = bload<T>(rin, bym, gofs(zs) + s * pbytes);
       Yp[s] = bload<T>(rin, byp, gofs(zs) + s * pbytes);
-      if (edge_lane) Xe[s] = bload<T>(rin, bedge, gofs(zs) + s * pbytes);
+      Xe[s] = bload<T>(rin, bxe, gofs(zs) + s * pbytes);
     }
   }
   I0 = bload_u8(rid, col, iofs(zs - 1));
   I1 = bload_u8(rid, col, iofs(zs));
   I2 = bload_u8(rid, col, iofs(zs + 1));
   I3 = bload_u8(rid, col, iofs(zs + 2));
-  I4 = bload_u8(rid, col, iofs(zs + 3));
   __builtin_amdgcn_s_waitcnt(0);
 #pragma unroll
   for (int s = 0; s < NS; s++) B0[s] = B1[s] = B2[s] = A1[s];
@@ -1336,31 +1345,47 @@ k_step2(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__
   /* One plane: level 1 at z from level 0 (am, ac, ap = planes z-1, z,
    * z+1; ym, yp, xe its neighbours at z), then level 2 at z-1 from level 1
    * (bm, bc, bn = planes z-2, z-1, z), both clamped at the grid's z faces
-   * (plane nz is plane nz-1 again). */
-  auto iter = [&](const int z, T (&am)[NS], T (&ac)[NS], T (&ap)[NS], T (&q0)[NS], T (&q1)[NS], T (&qn)[NS],
+   * (plane nz is plane nz-1 again).  Every plane, the one past the top
+   * face too, issues the same loads (clamped addresses): the level-0
+   * neighbours of plane z+1, then plane z+3.  Loads complete in issue
+   * order, so the next plane's wait for its neighbours does not take in
+   * plane z+3, which has two planes to arrive, and no path through the
+   * planes makes a wait cover loads that a later plane issued. */
+  auto iter = [&](const int z, T (&am)[NS], T (&ac)[NS], T (&ap)[NS], T (&q0)[NS], T (&qn)[NS], T (&qf)[NS],
                   T (&wm)[NS], T (&wc)[NS], T (&wn)[NS], uint32_t& idm, uint32_t& ic, uint32_t& iqn) {
     (void)q0;
-    (void)q1;
+    (void)qf;
     __syncthreads();
-    if (z < nz) {
-      /* in flight: plane z+4, then the level-0 neighbours of plane z+1 */
-#pragma unroll
-      for (int s = 0; s < NS; s++) qn[s] = bload<T>(rin, bc, gofs(z + 4) + s * pbytes);
-      iqn = bload_u8(rid, col, iofs(z + 4));
-      level(s1.dmask, s1.react, true, z, ic, am, ac, ap, Ym, Yp, Xe, wn);
-#pragma unroll
-      for (int s = 0; s < NS; s++)
-        if ((s1.dmask >> s) & 1u) {
-          Ym[s] = bload<T>(rin, bym, gofs(z + 1) + s * pbytes);
-          Yp[s] = bload<T>(rin, byp, gofs(z + 1) + s * pbytes);
-          if (edge_lane) Xe[s] = bload<T>(rin, bedge, gofs(z + 1) + s * pbytes);
-        }
-#pragma unroll
-      for (int s = 0; s < NS; s++)
-        if ((s2.dmask >> s) & 1u) ex[z & 1][s][wy][lane] = wn[s];
+    /* the step masks, read again from LDS at each plane (the barrier
+     * orders the first read after the write): each species test is then a
+     * bit test of one scalar register, not a flag hoisted out of the loop,
+     * of which there are too many to keep in scalar registers */
+    dm1 = __builtin_amdgcn_readfirstlane(msk[0]);
+    dm2 = __builtin_amdgcn_readfirstlane(msk[1]);
+    re1 = __builtin_amdgcn_readfirstlane(msk[2]);
+    re2 = __builtin_amdgcn_readfirstlane(msk[3]);
+    rbound = __builtin_amdgcn_readfirstlane(msk[4]);
+    const bool below = z < nz;
+    if (below) {
+      level(dm1, re1, true, z, ic, am, ac, ap, Ym, Yp, Xe, wn);
     } else {
 #pragma unroll
       for (int s = 0; s < NS; s++) wn[s] = wc[s];
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+      if ((dm1 >> s) & 1u) {
+        Ym[s] = bload<T>(rin, bym, gofs(z + 1) + s * pbytes);
+        Yp[s] = bload<T>(rin, byp, gofs(z + 1) + s * pbytes);
+        Xe[s] = bload<T>(rin, bxe, gofs(z + 1) + s * pbytes);
+      }
+#pragma unroll
+    for (int s = 0; s < NS; s++) qn[s] = bload<T>(rin, bc, gofs(z + 3) + s * pbytes);
+    iqn = bload_u8(rid, col, iofs(z + 3));
+    if (below) {
+#pragma unroll
+      for (int s = 0; s < NS; s++)
+        if ((dm2 >> s) & 1u) ex[z & 1][s][wy][lane] = wn[s];
     }
     const int zo = z - 1;
     if (zo >= z0) {
@@ -1370,12 +1395,12 @@ k_step2(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__
 #pragma unroll
       for (int s = 0; s < NS; s++) {
         Lm[s] = Lp[s] = T(0);
-        if ((s2.dmask >> s) & 1u) {
+        if ((dm2 >> s) & 1u) {
           Lm[s] = ex[zo & 1][s][wym][lane];
           Lp[s] = ex[zo & 1][s][wyp][lane];
         }
       }
-      level(s2.dmask, s2.react, false, zo, idm, wm, wc, wn, Lm, Lp, wc, nw);
+      level(dm2, re2, false, zo, idm, wm, wc, wn, Lm, Lp, wc, nw);
       if (inner) {
         const uint32_t so = (uint32_t)(zo - z0) * gbytes;
 #pragma unroll
@@ -1389,14 +1414,22 @@ k_step2(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__
     }
   };
 
+  /* Each plane after the first leaves the loop when it is past zt, so no
+   * path runs a later plane after skipping one. */
   const int zt = z1 == nz ? nz : z1;
-  for (int z = zs; z <= zt; z += 6) {
-    iter(z, A0, A1, A2, A3, A4, A5, B0, B1, B2, I0, I1, I5);
-    if (z + 1 <= zt) iter(z + 1, A1, A2, A3, A4, A5, A0, B1, B2, B0, I1, I2, I0);
-    if (z + 2 <= zt) iter(z + 2, A2, A3, A4, A5, A0, A1, B2, B0, B1, I2, I3, I1);
-    if (z + 3 <= zt) iter(z + 3, A3, A4, A5, A0, A1, A2, B0, B1, B2, I3, I4, I2);
-    if (z + 4 <= zt) iter(z + 4, A4, A5, A0, A1, A2, A3, B1, B2, B0, I4, I5, I3);
-    if (z + 5 <= zt) iter(z + 5, A5, A0, A1, A2, A3, A4, B2, B0, B1, I5, I0, I4);
+  for (int z = zs;; z += 6) {
+    iter(z, A0, A1, A2, A3, A4, A5, B0, B1, B2, I0, I1, I4);
+    if (z + 1 > zt) break;
+    iter(z + 1, A1, A2, A3, A4, A5, A0, B1, B2, B0, I1, I2, I5);
+    if (z + 2 > zt) break;
+    iter(z + 2, A2, A3, A4, A5, A0, A1, B2, B0, B1, I2, I3, I0);
+    if (z + 3 > zt) break;
+    iter(z + 3, A3, A4, A5, A0, A1, A2, B0, B1, B2, I3, I4, I1);
+    if (z + 4 > zt) break;
+    iter(z + 4, A4, A5, A0, A1, A2, A3, B1, B2, B0, I4, I5, I2);
+    if (z + 5 > zt) break;
+    iter(z + 5, A5, A0, A1, A2, A3, A4, B2, B0, B1, I5, I0, I3);
+    if (z + 6 > zt) break;
   }
 }
 

@@ -7,7 +7,9 @@ the same sequence of per-voxel updates as two one-step passes.  FP64 runs
 are held to the oracle at 1e-10 and to the one-step path at 1e-13, with
 identical channel trajectories; FP32 runs to the one-step path within a
 few FP32 roundings of the field's deviation from rest.  Grids cover tile
-edges in x (60 inner columns per tile) and y (12 rows in FP32, 4 in FP64),
+edges in x (62 inner columns per tile; nx = 63 and 125 put the last
+column on a tile's last lane, whose face neighbour is its own column) and
+y (14 rows in FP32, 6 in FP64),
 one- and two-plane grids, z chunks that do not divide the grid, odd step
 counts (a one-step pass at the end) and runs split into several calls.
 """
@@ -71,7 +73,8 @@ def _close(fu, other, rtol, what):
 
 
 @pytest.mark.parametrize("n,zchunk,xcd", [((20, 18, 14), 5, 0), ((65, 13, 9), 4, 1), ((130, 30, 6), 64, 8),
-                                          ((61, 16, 5), 1, 0), ((9, 9, 5), 3, 2)])
+                                          ((61, 16, 5), 1, 0), ((9, 9, 5), 3, 2), ((63, 12, 7), 2, 0),
+                                          ((125, 9, 6), 7, 1)])
 def test_fused_fp64_deterministic_matches_oracle(n, zchunk, xcd):
     d = dyad.small_dyad(n=n, seed=3)
     opts = cs.Options(seed=7, open={"ryr": [d.open_index, 2]}, t_close={"ryr": 1.0})
@@ -155,13 +158,14 @@ def test_fused_serca_and_dirichlet():
     _close(fu, one.species, 1e-13, "one-step passes")
 
 
-def test_fused_fp32_matches_one_step():
+@pytest.mark.parametrize("n,zchunk,xcd", [((130, 30, 12), 5, 8), ((63, 17, 9), 4, 0)])
+def test_fused_fp32_matches_one_step(n, zchunk, xcd):
     """FP32: the fused pass applies the same per-voxel updates in the same
     order, so it agrees with one-step passes to FP32 rounding of the
     stored deviations."""
-    d = dyad.small_dyad(n=(130, 30, 12), seed=6)
+    d = dyad.small_dyad(n=n, seed=6)
     opts = cs.Options(seed=5, open={"ryr": [d.open_index]}, t_close={"ryr": 1.0}, precision="f32")
-    fu, one, orc = _run(d, opts, [100, 101], zchunk=5, xcd=8)
+    fu, one, orc = _run(d, opts, [100, 101], zchunk=zchunk, xcd=xcd)
     _close(fu, orc.species, 2e-4, "oracle")
     for s in range(fu.ns):
         a, b = fu.species(s), one.species(s)
