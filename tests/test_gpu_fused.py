@@ -112,6 +112,33 @@ def test_fused_thin_slabs(nz, zchunk):
     _close(fu, one.species, 1e-13, "one-step passes")
 
 
+@pytest.mark.parametrize("n,precision", [((63, 12, 7), "f64"), ((125, 9, 6), "f64"), ((63, 17, 8), "f32")])
+def test_fused_face_on_last_lane(n, precision):
+    """nx = 62k + 1 puts the grid's last column on a tile's last lane, whose
+    x neighbour across the face is its own value on every plane.  The fields
+    start perturbed by 20%, so the faces carry gradients from the first
+    step and a stale or wrong neighbour there shows at once."""
+    d = dyad.small_dyad(n=n, seed=3)
+    opts = cs.Options(seed=7, open={"ryr": [d.open_index]}, t_close={"ryr": 1.0}, precision=precision)
+    fu, one, orc = _run(d, opts, [0], zchunk=3, xcd=0)
+    x = [orc.species(s) * (1 + 0.2 * np.random.default_rng(s).random(orc.species(s).shape)) for s in range(orc.ns)]
+    for sim in (fu, one, orc):
+        for s in range(orc.ns):
+            sim.set_species(s, x[s])
+    for k in (30, 31):
+        fu.step(k)
+        one.step(k)
+        orc.step(k)
+    if precision == "f64":
+        _close(fu, orc.species, RTOL_F64, "oracle")
+        _close(fu, one.species, 1e-13, "one-step passes")
+    else:
+        _close(fu, orc.species, 2e-4, "oracle")
+        for s in range(fu.ns):
+            a, b = fu.species(s), one.species(s)
+            assert float(np.max(np.abs(a - b))) <= 1e-6 * float(np.max(np.abs(b))), f"species {s}"
+
+
 def test_fused_forced_close_and_substeps():
     """-C closes the forced-open channel mid-run; reaction and gating
     substeps > 1, so some pairs gate or react on one step only."""
