@@ -3120,10 +3120,13 @@ int cs_autotune(cs_context* c, uint64_t first_time_ind, uint32_t reps, cs_launch
     if (nz <= fzmax) fzs.push_back(nz);
     if (std::find(fzs.begin(), fzs.end(), c->launch.fz) == fzs.end() && c->launch.fz <= fzmax)
       fzs.push_back(c->launch.fz);   /* the default's balanced chunk */
+    std::vector<int> fxs{0, 1, 8};
+    if (std::find(fxs.begin(), fxs.end(), c->launch.fx) == fxs.end())
+      fxs.push_back(c->launch.fx);   /* the default's tile groups */
     double fms = 1e30;
     LaunchCfg fbest = winner;
     for (int zc : fzs)
-      for (int fx : {0, 1, 8}) {
+      for (int fx : fxs) {
         LaunchCfg L = winner;
         L.fuse = 1;
         L.fz = zc;
