@@ -1889,13 +1889,14 @@ LaunchCfg default_launch(const Geo& g, size_t esize, int cus)
     if (cost < best - 1e-9) { best = cost; L.fz = zc; }
   }
   L.fz = std::max(1, std::min(L.fz, std::max(fzmax, 1)));
-  /* XCD groups of up to 16 consecutive tiles (16 measured about 1% faster
-   * than 8 on the 400x400x200 dyad), small enough that every XCD gets at
-   * least four groups (hardware order for small grids) */
+  /* XCD groups of up to 32 consecutive tiles, small enough that every
+   * XCD gets at least three groups (hardware order for small grids).  On
+   * the 400x400x200 dyad (1015 blocks) groups of 32, 16 and 8 take 501,
+   * 508 and 512 us per pass; 64 takes 533 */
   const long nb = tiles * ((g.nz + L.fz - 1) / L.fz);
   L.fx = 0;
-  for (int G = 16; G >= 2; G /= 2)
-    if (nb >= 8L * 4 * G) { L.fx = G; break; }
+  for (int G = 32; G >= 2; G /= 2)
+    if (nb >= 8L * 3 * G) { L.fx = G; break; }
   return L;
 }
 
