@@ -1343,8 +1343,8 @@ k_step2(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__
   for (int s = 0; s < NS; s++) B0[s] = B1[s] = B2[s] = A1[s];
 
   /* One plane: level 1 at z from level 0 (am, ac, ap = planes z-1, z,
-   * z+1; ym, yp, xe its neighbours at z), then level 2 at z-1 from level 1
-   * (bm, bc, bn = planes z-2, z-1, z), both clamped at the grid's z faces
+   * z+1; Ym, Yp, Xe its neighbours at z), then level 2 at z-1 from level 1
+   * (wm, wc, wn = planes z-2, z-1, z), both clamped at the grid's z faces
    * (plane nz is plane nz-1 again).  Every plane, the one past the top
    * face too, issues the same loads (clamped addresses): the level-0
    * neighbours of plane z+1, then plane z+3.  Loads complete in issue
