@@ -182,9 +182,12 @@ int cs_time_passes(cs_context* ctx, uint64_t first_time_ind, uint64_t nsteps, do
                    double* pass_ms, uint64_t* passes, uint64_t* fused);
 
 /* 1 when the problem admits fused two-step passes, else 0 with the
- * reason in why (if given).  They need boundary pairs that are face
- * neighbours, at most one channel per voxel, at most 7 species and fewer
- * than 65536 planes.  No reference counterpart. */
+ * reason in why (if given).  They need at most 7 species, every boundary
+ * flux acting on species 0, at most 16384 flux rows (species, voxel) and
+ * at most 16384 channels (one k_bridge block gates and sums them all),
+ * and a z-group of every species of a plane small enough that the
+ * pass's 32-bit buffer offsets cover at least one plane plus its four
+ * neighbour planes.  No reference counterpart. */
 int cs_fused_available(const cs_context* ctx, char* why, size_t why_len);
 
 /* Launch shape of the fused stencil/reaction kernel: blocks of 64 x rows

@@ -129,7 +129,13 @@ def test_cli_mt_stream_matches_oracle(tmp_path):
 
 def test_cli_abort_after_channels_close(tmp_path):
     """-X: stop tabort ms after an output first shows every channel closed
-    (species.c:3384-3488); fewer outputs than the full run."""
+    (species.c:3384-3488).  At 6 nm dt = 0.95*36/(6*220000) ms, so -o 0.002
+    saves every floor(0.002/dt) = 77 steps and -t 0.02 runs 772 steps: 11
+    outputs with the one at t = 0.  -C 0.001 closes both channels at the
+    gating step with ti dt + dt >= 0.001, ti = 38; the output at step 77 is
+    the first to see them closed and starts the clock at 77 dt; the check
+    after step k fires once (k - 77) dt >= 0.003, at k = 193, after the
+    outputs at steps 0, 77 and 154 only."""
     d = dyad.small_dyad(seed=4)
     gp, mp = _inputs(tmp_path, d)
     common = ["-g", gp, "-m", mp, "-t", "0.02", "-o", "0.002", "-S", "1", "-O", "ryr", "0", "p_ryr", "0",
@@ -138,4 +144,4 @@ def test_cli_abort_after_channels_close(tmp_path):
     ab = _run_cli(common + ["-c", "ab", "-X", "0.003"], tmp_path)
     n_full = full.stdout.count(": time:")
     n_ab = ab.stdout.count(": time:")
-    assert 1 < n_ab < n_full
+    assert (n_full, n_ab) == (11, 3), (n_full, n_ab)
