@@ -2,7 +2,8 @@
 
 Produces, under ``_build/`` next to this file:
 
-* ``libcsparks.so``  -- HIP kernels + the cs_* C ABI (hipcc, gfx950)
+* ``libcsparks.so``  -- HIP kernels + the cs_* C ABI (hipcc, gfx950):
+  single domains (csparks.hip) and ensembles (cs_batch.hip)
 * ``libcshost.so``   -- problem builder, Mersenne Twister, loop rules (gcc)
 * ``libcsio.so``     -- HDF5 reader/writer for the reference file formats
 * ``calcium_sparks`` -- the drop-in command-line program
@@ -53,7 +54,7 @@ def build_device():
     out = os.path.join(OUT, "libcsparks.so")
     tmp = out + ".tmp"
     _run([_hipcc(), *HIPCC_FLAGS, "-shared", "-I", INC, "-I", CSRC,
-          os.path.join(CSRC, "csparks.hip"), "-o", tmp])
+          os.path.join(CSRC, "csparks.hip"), os.path.join(CSRC, "cs_batch.hip"), "-o", tmp])
     os.replace(tmp, out)
     return out
 

@@ -180,6 +180,14 @@ class CLaunchConfig(C.Structure):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
+class CBatchLaunch(C.Structure):
+    _fields_ = [("rows", C.c_uint32), ("zchunk", C.c_uint32), ("prefetch", C.c_uint32),
+                ("nontemporal", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
 class CAbortState(C.Structure):
     _fields_ = [("tabort", C.c_double), ("last_closed", C.c_double),
                 ("all_closed", C.c_int), ("active", C.c_int)]
@@ -270,6 +278,29 @@ def device_lib() -> C.CDLL:
         lib.cs_autotune.argtypes = [P, C.c_uint64, C.c_uint32, C.POINTER(CLaunchConfig), C.POINTER(C.c_double)]
         lib.cs_calibration_copy.argtypes = [C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]
         lib.cs_last_error.restype = C.c_char_p
+        pd = C.POINTER(C.c_double)
+        lib.cs_batch_create.argtypes = [C.POINTER(CProblem), C.c_uint32, C.POINTER(P)]
+        lib.cs_batch_destroy.argtypes = [P]
+        lib.cs_batch_reset.argtypes = [P]
+        lib.cs_batch_info.argtypes = [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        lib.cs_batch_upload_species.argtypes = [P, C.c_uint32, C.c_uint32, pd]
+        lib.cs_batch_download_species.argtypes = [P, C.c_uint32, C.c_uint32, pd]
+        lib.cs_batch_step.argtypes = [P, C.c_uint64, C.c_uint64]
+        lib.cs_batch_step_uniforms.argtypes = [P, C.c_uint64, C.c_uint64, pd, C.c_uint64]
+        lib.cs_batch_uniforms_needed.argtypes = [P, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64)]
+        lib.cs_batch_synchronize.argtypes = [P]
+        lib.cs_batch_reduce_stats.argtypes = [P, pd, pd, pd]
+        lib.cs_batch_linescan.argtypes = [P, C.c_uint32, C.c_uint32, pd, C.c_uint32, pd]
+        lib.cs_batch_get_channel_states.argtypes = [P, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint8)]
+        lib.cs_batch_get_channel_recruited.argtypes = [P, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint8)]
+        lib.cs_batch_channel_summary.argtypes = [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
+                                                 C.POINTER(C.c_uint32)]
+        lib.cs_batch_set_active.argtypes = [P, C.POINTER(C.c_uint8)]
+        lib.cs_batch_get_launch.argtypes = [P, C.POINTER(CBatchLaunch)]
+        lib.cs_batch_set_launch.argtypes = [P, C.POINTER(CBatchLaunch)]
+        lib.cs_batch_autotune.argtypes = [P, C.c_uint64, C.c_uint32, C.POINTER(CBatchLaunch), pd]
+        lib.cs_batch_set_graph.argtypes = [P, C.c_uint32]
+        lib.cs_batch_time_steps.argtypes = [P, C.c_uint64, C.c_uint64, pd, pd]
         lib._cs_typed = True
     return lib
 
@@ -699,8 +730,9 @@ class Setup:
 class Simulation:
     """A resident calcium_sparks domain on the current GPU."""
 
-    def __init__(self, geom: Geometry, model: SpeciesModel, opts: Options):
+    def __init__(self, geom: Geometry, model: SpeciesModel, opts: Options, instance: int = 0):
         self.setup = Setup(geom, model, opts)
+        self.setup.s.problem.instance = int(instance)   # Philox key word 1
         self.opts = opts
         self.ctx = C.c_void_p()
         _check(device_lib().cs_create(C.byref(self.setup.s.problem), C.byref(self.ctx)), "cs_create")
@@ -836,6 +868,207 @@ class Simulation:
             self.ctx = C.c_void_p()
         if getattr(self, "setup", None) is not None:
             self.setup.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Ensemble:
+    """Realisations of one species model stepped together on the current GPU
+    (cs_batch_*): each member is a (geometry, model, options) triple whose
+    model and options may differ only in what the geometry, the channel
+    layout and the seed decide.  Member i takes the same steps as a
+    Simulation of the same triple (its Philox key is the seed its setup
+    chose and the problem's instance word)."""
+
+    def __init__(self, members: Sequence[Tuple[Geometry, SpeciesModel, Options]],
+                 instances: Optional[Sequence[int]] = None):
+        if not members:
+            raise ValueError("an ensemble needs at least one member")
+        self.setups = [Setup(g, m, o) for g, m, o in members]
+        if instances is not None:
+            for s, k in zip(self.setups, instances):
+                s.s.problem.instance = int(k)
+        arr = (CProblem * len(self.setups))(*[s.s.problem for s in self.setups])
+        self.handle = C.c_void_p()
+        _check(device_lib().cs_batch_create(arr, len(self.setups), C.byref(self.handle)), "cs_batch_create")
+        self.count = len(self.setups)
+        self.time_ind = 1
+        self.ns = int(self.setups[0].s.num_species)
+        self.nd = int(self.setups[0].s.num_domains)
+        self.mt = self.setups[0].problem.rng == RNG_MT
+
+    @property
+    def dt(self):
+        return self.setups[0].dt
+
+    def n(self, i: int):
+        return self.setups[i].n
+
+    def step(self, nsteps: int = 1):
+        lib = device_lib()
+        if self.mt:
+            # each member's Mersenne Twister draws, gating step by gating
+            # step, members one after the other within a gating step
+            per = [s.mt_uniforms(self.time_ind, nsteps) for s in self.setups]
+            draws = self.draws_per_gate()
+            total = sum(draws)
+            cnt = C.c_uint64()
+            _check(lib.cs_batch_uniforms_needed(self.handle, self.time_ind, nsteps, C.byref(cnt)),
+                   "cs_batch_uniforms_needed")
+            gates = cnt.value // total if total else 0
+            uni = (np.ascontiguousarray(np.hstack([p.reshape(gates, k) for p, k in zip(per, draws)]).ravel())
+                   if gates else np.zeros(0))
+            _check(lib.cs_batch_step_uniforms(self.handle, self.time_ind, nsteps,
+                                              uni.ctypes.data_as(C.POINTER(C.c_double)), len(uni)),
+                   "cs_batch_step_uniforms")
+        else:
+            _check(lib.cs_batch_step(self.handle, self.time_ind, nsteps), "cs_batch_step")
+        self.time_ind += nsteps
+
+    def draws_per_gate(self) -> List[int]:
+        """Gating uniforms each member draws per gating step (its channels
+        without a forced closing time)."""
+        out = []
+        for s in self.setups:
+            p = s.problem
+            out.append(sum(int(p.fluxes[f].num_pairs) for f in range(p.num_fluxes)
+                           if p.fluxes[f].discrete and not p.fluxes[f].params[8] > 0))
+        return out
+
+    def synchronize(self):
+        _check(device_lib().cs_batch_synchronize(self.handle), "cs_batch_synchronize")
+
+    def species(self, i: int, s: int) -> np.ndarray:
+        out = np.empty(self.n(i), dtype=np.float64)
+        _check(device_lib().cs_batch_download_species(self.handle, i, s, out.ctypes.data_as(C.POINTER(C.c_double))),
+               "cs_batch_download_species")
+        return out
+
+    def set_species(self, i: int, s: int, values: np.ndarray):
+        arr = np.ascontiguousarray(values, dtype=np.float64).reshape(self.n(i))
+        _check(device_lib().cs_batch_upload_species(self.handle, i, s, arr.ctypes.data_as(C.POINTER(C.c_double))),
+               "cs_batch_upload_species")
+
+    def stats(self):
+        """Per member (min, max, total) arrays of shape [count, nd, ns]."""
+        k = self.count * self.nd * self.ns
+        mn, mx, tot = np.empty(k), np.empty(k), np.empty(k)
+        ptr = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+        _check(device_lib().cs_batch_reduce_stats(self.handle, ptr(mn), ptr(mx), ptr(tot)), "cs_batch_reduce_stats")
+        shp = (self.count, self.nd, self.ns)
+        return mn.reshape(shp), mx.reshape(shp), tot.reshape(shp)
+
+    def linescans(self, species: int, axis: int, offsets, conv, num_domains: int = 0):
+        """Simulation.linescan of every member (one device pass for all)."""
+        w, ks = [], []
+        for i in range(self.count):
+            n = self.n(i)
+            h = self.setups[i].problem.h
+            off = [(n[ax] // 2 + int(offsets[ax])) * h for ax in range(3)]
+            w.extend(gauss(off[ax] - np.arange(n[ax]) * h, conv[ax]) for ax in range(3))
+            ks.append(n[axis])
+        weights = np.ascontiguousarray(np.concatenate(w))
+        proj = np.empty(sum(ks))
+        mask = 0xFFFFFFFF if not num_domains else (0xFF if num_domains >= 8 else (1 << num_domains) - 1)
+        ptr = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+        _check(device_lib().cs_batch_linescan(self.handle, species, axis, ptr(weights), mask, ptr(proj)),
+               "cs_batch_linescan")
+        out, o = [], 0
+        for i, k in enumerate(ks):
+            h = self.setups[i].problem.h
+            pos = np.arange(k) * h
+            g = gauss(pos[:, None] - pos[None, :], conv[axis])
+            out.append(g @ proj[o:o + k])
+            o += k
+        return out
+
+    def _flux_size(self, i: int, flux: int) -> int:
+        return int(self.setups[i].problem.fluxes[flux].num_pairs)
+
+    def channel_states(self, i: int, flux: int) -> np.ndarray:
+        n = self._flux_size(i, flux)
+        out = np.zeros(max(n, 1), dtype=np.uint8)
+        _check(device_lib().cs_batch_get_channel_states(self.handle, i, flux, out.ctypes.data_as(C.POINTER(C.c_uint8))),
+               "cs_batch_get_channel_states")
+        return out[:n]
+
+    def recruited(self, i: int, flux: int) -> np.ndarray:
+        n = self._flux_size(i, flux)
+        out = np.zeros(max(n, 1), dtype=np.uint8)
+        _check(device_lib().cs_batch_get_channel_recruited(self.handle, i, flux,
+                                                           out.ctypes.data_as(C.POINTER(C.c_uint8))),
+               "cs_batch_get_channel_recruited")
+        return out[:n]
+
+    def summary(self):
+        """(open channels, time index of the latest gating step with a channel
+        open, channels recruited) per member."""
+        no = np.zeros(self.count, dtype=np.uint32)
+        lo = np.zeros(self.count, dtype=np.uint64)
+        nr = np.zeros(self.count, dtype=np.uint32)
+        _check(device_lib().cs_batch_channel_summary(self.handle, no.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                                     lo.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                                     nr.ctypes.data_as(C.POINTER(C.c_uint32))),
+               "cs_batch_channel_summary")
+        return no, lo, nr
+
+    def set_active(self, active):
+        arr = np.ascontiguousarray(np.asarray(active, dtype=bool).astype(np.uint8))
+        if arr.size != self.count:
+            raise ValueError("one flag per member")
+        _check(device_lib().cs_batch_set_active(self.handle, arr.ctypes.data_as(C.POINTER(C.c_uint8))),
+               "cs_batch_set_active")
+
+    def launch(self) -> dict:
+        cfg = CBatchLaunch()
+        _check(device_lib().cs_batch_get_launch(self.handle, C.byref(cfg)), "cs_batch_get_launch")
+        return cfg.as_dict()
+
+    def set_launch(self, **kw):
+        cfg = CBatchLaunch(**{**self.launch(), **kw})
+        _check(device_lib().cs_batch_set_launch(self.handle, C.byref(cfg)), "cs_batch_set_launch")
+
+    def autotune(self, reps: int = 2):
+        cfg, ms = CBatchLaunch(), C.c_double()
+        _check(device_lib().cs_batch_autotune(self.handle, self.time_ind, reps, C.byref(cfg), C.byref(ms)),
+               "cs_batch_autotune")
+        return cfg.as_dict(), ms.value
+
+    def set_graph(self, on: bool):
+        _check(device_lib().cs_batch_set_graph(self.handle, 1 if on else 0), "cs_batch_set_graph")
+
+    def time_steps(self, nsteps: int):
+        tot, ker = C.c_double(), C.c_double()
+        _check(device_lib().cs_batch_time_steps(self.handle, self.time_ind, nsteps, C.byref(tot), C.byref(ker)),
+               "cs_batch_time_steps")
+        self.time_ind += nsteps
+        return tot.value, ker.value
+
+    def info(self):
+        cnt, nv, nb = C.c_uint32(), C.c_uint64(), C.c_uint64()
+        _check(device_lib().cs_batch_info(self.handle, C.byref(cnt), C.byref(nv), C.byref(nb)), "cs_batch_info")
+        return int(cnt.value), int(nv.value), int(nb.value)
+
+    def reset(self):
+        _check(device_lib().cs_batch_reset(self.handle), "cs_batch_reset")
+        self.time_ind = 1
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            device_lib().cs_batch_destroy(self.handle)
+            self.handle = C.c_void_p()
+        for s in getattr(self, "setups", []):
+            s.close()
 
     def __enter__(self):
         return self

@@ -97,8 +97,10 @@ class DyadLayout:
     footprint: Tuple[int, int, int, int]  # x0, x1, y0, y1 of the jSR/cleft
 
 
-def dyad_geometry(layout: DyadLayout, ryr_xy: Sequence[Tuple[int, int]], pryr_mask: Sequence[bool]) -> Geometry:
-    """Voxel geometry + discrete 'ryr' / 'p_ryr' boundaries (jSR -> cleft)."""
+def dyad_geometry(layout: DyadLayout, ryr_xy: Sequence[Tuple[int, int]], pryr_mask: Sequence[bool],
+                  always_pryr: bool = False) -> Geometry:
+    """Voxel geometry + discrete 'ryr' / 'p_ryr' boundaries (jSR -> cleft);
+    with always_pryr the 'p_ryr' boundary exists even without channels."""
     nx, ny, nz = layout.n
     vox = np.full((nx, ny, nz), CYT, dtype=np.uint8)
     vox[:, :, :layout.tt] = TT
@@ -113,7 +115,7 @@ def dyad_geometry(layout: DyadLayout, ryr_xy: Sequence[Tuple[int, int]], pryr_ma
             raise ValueError("channel outside the jSR footprint")
         (pryr if p else ryr).append((x, y, zj0, x, y, zc1 - 1))
     bnds = [Boundary("ryr", "discrete", np.array(ryr, dtype=np.uint16).reshape(-1, 6))]
-    if pryr:
+    if pryr or always_pryr:
         bnds.append(Boundary("p_ryr", "discrete", np.array(pryr, dtype=np.uint16).reshape(-1, 6)))
     return Geometry(voxels=vox, h=layout.h, domain_names=list(DOMAINS),
                     domain_connections=connections(), boundaries=bnds)
@@ -230,3 +232,53 @@ def fast_spark_dyad(seed: int = 3, num_ryr: int = 16, num_pryr: int = 4) -> Dyad
     model = dataclasses.replace(m, ryr=dataclasses.replace(m.ryr, **fast),
                                 p_ryr=dataclasses.replace(m.p_ryr, **fast) if m.p_ryr else None)
     return Dyad(d.geometry, model, d.open_index, f"fast-gating {d.description}")
+
+
+def ensemble_dyad(seed: int, h: float = 6.0, edge_nm: Tuple[float, float] = (256.0, 512.0),
+                  ryr: Tuple[int, int] = (9, 100), pryr_max: float = 0.5, depth_nm: float = 150.0) -> Dyad:
+    """One realisation of the ensemble workload, drawn from `seed`: a square
+    dyad of edge L ~ U[edge_nm] and depth depth_nm at voxel size h, with
+    the lattice dyad's layers (T-tubule 30 nm, cleft 12 nm, jSR 30 nm); a
+    cluster of N ~ U{ryr} RyRs by Poisson-disk sampling in a disk sized to
+    about one RyR per (27 nm)^2, the spacing relaxed from 25 nm until all
+    fit; a fraction ~ U[0, pryr_max] of them phosphorylated (the p_ryr
+    boundary exists even when empty, so every realisation has the same
+    fluxes); the jSR/cleft footprint covers the cluster and at least 62.5%
+    of the edge.  open_index is the non-phosphorylated RyR nearest the
+    cluster centre, the one that starts the spark."""
+    rng = np.random.default_rng(seed)
+    L = float(rng.uniform(*edge_nm))
+    nxy = max(8, int(round(L / h)))
+    nz = max(6, int(round(depth_nm / h)))
+    count = int(rng.integers(ryr[0], ryr[1] + 1))
+    frac = float(rng.uniform(0.0, pryr_max))
+    tt, cleft, jsr = max(1, int(round(30.0 / h))), max(1, int(round(12.0 / h))), max(1, int(round(30.0 / h)))
+    if tt + cleft + jsr + 1 > nz:
+        raise ValueError("dyad too shallow for its layers")
+    c = nxy // 2
+    radius_nm = min(27.0 * math.sqrt(count / math.pi) * 1.15, 0.5 * (nxy - 4) * h)
+    pts = None
+    for spacing in (25.0, 22.0, 19.0, 16.0, 13.0, 10.0, 7.0, h):
+        try:
+            pts = poisson_disk(rng, count, radius_vox=radius_nm / h, min_dist_vox=max(1.0, spacing / h),
+                               center=(c, c), max_tries=4000)
+            break
+        except RuntimeError:
+            continue
+    if pts is None:
+        raise RuntimeError("cannot place the cluster")
+    half = max(int(math.ceil(radius_nm / h)) + 2, int(round(0.625 * nxy / 2)))
+    half = min(half, c - 1, nxy - c - 1)
+    lay = DyadLayout(n=(nxy, nxy, nz), h=h, tt=tt, cleft=cleft, jsr=jsr,
+                     footprint=(c - half, c + half, c - half, c + half))
+    npryr = int(round(frac * count))
+    dist = [(x - c) ** 2 + (y - c) ** 2 for x, y in pts]
+    trigger = int(np.argmin(dist))
+    cand = [i for i in range(count) if i != trigger]
+    chosen = set(rng.choice(cand, size=npryr, replace=False).tolist()) if npryr else set()
+    mask = [i in chosen for i in range(count)]
+    geom = dyad_geometry(lay, pts, mask, always_pryr=True)
+    ryr_idx = [i for i in range(count) if not mask[i]]
+    model = dyad_species_model(with_pryr=True)
+    return Dyad(geom, model, ryr_idx.index(trigger),
+                f"ensemble dyad {nxy}x{nxy}x{nz} @{h}nm, {count} RyR ({npryr} pRyR)")

@@ -238,6 +238,85 @@ int cs_autotune(cs_context* ctx, uint64_t first_time_ind, uint32_t reps,
  * memory-traffic counters.  No reference counterpart. */
 int cs_calibration_copy(uint64_t bytes, uint32_t reps, double* ms_per_copy);
 
+/* ---- ensembles -----------------------------------------------------------
+ *
+ * An ensemble steps many independent realisations of one species model
+ * together on the device.  The reference runs one realisation per
+ * process: the seed is set once (calcium_sparks.c:113-116), the loop
+ * (calcium_sparks.c:190-228) advances that one domain, and ensembles are
+ * separate jobs (Run_scripts/G1/nonP/run_G1_nonP_0_sr900_ryr55.sh:61).
+ * Every realisation keeps its own grid size, domain ids, boundary pairs,
+ * initial channel states and Philox key (seed, instance); everything else
+ * in its cs_problem (species, dt, tables, flux models, substeps,
+ * precision, generator) must equal the first problem's.  A realisation
+ * of an ensemble takes the same steps, in the same arithmetic, as a
+ * cs_context built from its cs_problem.
+ *
+ * Per-realisation arrays are packed in realisation order: field transfers
+ * and channel queries name the realisation; cs_batch_reduce_stats fills
+ * [count][num_domains][num_species]. */
+typedef struct cs_batch cs_batch;
+
+int cs_batch_create(const cs_problem* problems, uint32_t count, cs_batch** out);
+int cs_batch_destroy(cs_batch* b);
+/* Fields back to rest, channels to their initial states, all realisations
+ * active (Species_init_values per realisation, species.c:887-935). */
+int cs_batch_reset(cs_batch* b);
+/* Realisations, voxels over all of them, and device bytes of the fields
+ * (two stacks) plus domain ids. */
+int cs_batch_info(const cs_batch* b, uint32_t* count, uint64_t* voxels, uint64_t* field_bytes);
+int cs_batch_upload_species(cs_batch* b, uint32_t realisation, uint32_t species, const double* src);
+int cs_batch_download_species(cs_batch* b, uint32_t realisation, uint32_t species, double* dst);
+/* cs_step / cs_step_uniforms for every active realisation.  In CS_RNG_MT
+ * mode the uniforms of each gating step are the realisations' draws (as
+ * cs_step_uniforms orders them) one realisation after the other. */
+int cs_batch_step(cs_batch* b, uint64_t first_time_ind, uint64_t nsteps);
+int cs_batch_step_uniforms(cs_batch* b, uint64_t first_time_ind, uint64_t nsteps,
+                           const double* uniforms, uint64_t count);
+int cs_batch_uniforms_needed(const cs_batch* b, uint64_t first_time_ind, uint64_t nsteps,
+                             uint64_t* count);
+int cs_batch_synchronize(cs_batch* b);
+/* cs_reduce_stats of every realisation: [count][num_domains][num_species]. */
+int cs_batch_reduce_stats(cs_batch* b, double* min_values, double* max_values, double* total_values);
+/* cs_linescan of every realisation at once.  weights holds, per
+ * realisation, its n[0] + n[1] + n[2] axis weights (x, then y, then z);
+ * proj receives, per realisation, n[axis] values. */
+int cs_batch_linescan(cs_batch* b, uint32_t species, uint32_t axis, const double* weights,
+                      uint32_t domain_mask, double* proj);
+int cs_batch_get_channel_states(cs_batch* b, uint32_t realisation, uint32_t flux, uint8_t* states);
+int cs_batch_get_channel_recruited(cs_batch* b, uint32_t realisation, uint32_t flux, uint8_t* recruited);
+/* Per realisation: channels open after its latest gating step, the time
+ * index of the latest gating step that left a channel open (0: none
+ * since cs_batch_reset), and channels recruited so far.  Any pointer may
+ * be NULL.  This is what the -X rule (species.c:3384-3488) needs to end a
+ * realisation's spark. */
+int cs_batch_channel_summary(cs_batch* b, uint32_t* nopen, uint64_t* last_open_time_ind,
+                             uint32_t* nrecruited);
+/* Keep stepping only the realisations with active[i] != 0; the others
+ * keep their state.  The stencil pass drops their tiles. */
+int cs_batch_set_active(cs_batch* b, const uint8_t* active);
+
+/* Launch shape of the ensemble's stencil pass: blocks of 64 x rows
+ * threads take tiles of 64*rows consecutive plane columns of one
+ * realisation by zchunk planes from eight per-XCD work queues; prefetch
+ * and nontemporal as in cs_launch_config.  No reference counterpart. */
+typedef struct cs_batch_launch {
+  uint32_t rows, zchunk, prefetch, nontemporal;
+} cs_batch_launch;
+int cs_batch_get_launch(const cs_batch* b, cs_batch_launch* cfg);
+int cs_batch_set_launch(cs_batch* b, const cs_batch_launch* cfg);
+/* Time every candidate shape on one period of the step pattern from
+ * first_time_ind (state unchanged) and keep the fastest. */
+int cs_batch_autotune(cs_batch* b, uint64_t first_time_ind, uint32_t reps, cs_batch_launch* best,
+                      double* best_ms);
+/* on != 0: cs_batch_step replays whole periods of the step pattern (the
+ * least common multiple of every substep, doubled if odd) as captured HIP
+ * graphs, with Philox gating.  No reference counterpart. */
+int cs_batch_set_graph(cs_batch* b, uint32_t on);
+/* Time nsteps steps: total ms, and the mean stencil pass (0 with graphs). */
+int cs_batch_time_steps(cs_batch* b, uint64_t first_time_ind, uint64_t nsteps, double* total_ms,
+                        double* stencil_ms);
+
 /* Scalar flux functions of the plugin table, evaluated on the host with
  * the same code the device runs (ryr_flux boundaryfluxes.c:87-141,
  * serca_flux :12-85). */

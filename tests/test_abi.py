@@ -47,3 +47,19 @@ def test_error_paths_without_gpu():
     assert lib.cs_create(None, C.byref(ctx)) == -1
     assert b"NULL" in lib.cs_last_error()
     assert lib.cs_step(None, 1, 1) == -1
+
+
+def test_ensemble_rejects_mixed_models_without_gpu():
+    """cs_batch_create checks that every realisation shares the first one's
+    species model before it touches the device."""
+    import dyad
+    lib = cs.device_lib()
+    b = C.c_void_p()
+    assert lib.cs_batch_create(None, 0, C.byref(b)) == -1
+    d1, d2 = dyad.small_dyad(), dyad.small_dyad(n=(16, 16, 12), h=3.0)
+    o = cs.Options(seed=1, open={"ryr": [0]})
+    s1, s2 = cs.Setup(d1.geometry, d1.model, o), cs.Setup(d2.geometry, d2.model, o)
+    arr = (cs.CProblem * 2)(s1.s.problem, s2.s.problem)
+    assert lib.cs_batch_create(arr, 2, C.byref(b)) == -1
+    assert b"realisation 1" in lib.cs_last_error() and b"differs" in lib.cs_last_error()
+    assert lib.cs_batch_step(None, 1, 1) == -1
