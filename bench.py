@@ -1,12 +1,13 @@
 #!/usr/bin/env python3
 """Benchmark: calcium_sparks time stepping on one MI355X.
 
-Workload (the single-instance configuration): a 1200x1200x600 nm dyad at
-3 nm (400x400x200 = 32M voxels), 5 species slots (7 species), cleft 12 nm,
-jSR 30 nm over 600x600 nm, 100 RyRs by seeded Poisson-disk sampling in a
-400 nm disk (30% pRyR), stochastic Philox gating, FP32 fields.  A step is
-one full time step of the scheme (gating, stencil + reaction, boundary
-fluxes).  Inputs are synthetic and resident in HBM before timing starts.
+Headline workload (the single-instance configuration): a 1200x1200x600 nm
+dyad at 3 nm (400x400x200 = 32M voxels), 5 species slots (7 species),
+cleft 12 nm, jSR 30 nm over 600x600 nm, 100 RyRs by seeded Poisson-disk
+sampling in a 400 nm disk (30% pRyR), stochastic Philox gating, FP32
+fields.  A step is one full time step of the scheme (gating, stencil +
+reaction, boundary fluxes).  Inputs are synthetic and resident in HBM
+before timing starts.
 
 Prints one JSON line.  `value` is the whole-step algorithmic HBM
 bandwidth: 41 B per voxel-step (1 B domain id + 5 species x (4 B read +
@@ -14,10 +15,21 @@ bandwidth: 41 B per voxel-step (1 B domain id + 5 species x (4 B read +
 alone (HIP events on its stream): a one-step pass (k_stencil) or a fused
 two-step pass (k_step2, the default where the problem admits it), priced
 at 41 B per voxel-step times the voxel-steps one launch processes (one
-or two per voxel).  `traffic` is the HBM bytes per launch the committed
-rocprofv3 counters measured for the same launch shape.  `cpu_baseline`
-runs the FP64 C oracle on the host cores for a bounded sample of the same
-workload.
+or two per voxel).  `traffic` is the HBM bytes per launch that committed
+rocprofv3 counters measured for the same launch shape and kernel source
+(null when the kernel source changed since).  `cpu_baseline` runs the
+FP64 C oracle on the host cores for a bounded sample of the same workload.
+
+Sub-records of the same line:
+  fp64      the same dyad with FP64 fields (one-step passes), the
+            reference's own precision;
+  ensemble  512 realisations of 256-512 nm dyads at 6 nm (9-100 RyRs,
+            0-50% pRyR) stepped as one ensemble: spark realisations/s over
+            their 20 ms runs;
+  closure   realisations/s of 256 dyads of 512-1024 nm (25-150 RyRs) at 12
+            nm with the learned closure gating, against the plain ensemble
+            of the same dyads at 3 nm, over their 50 ms runs.
+Realisations/s = realisations / (steps of one run x measured step time).
 """
 from __future__ import annotations
 
@@ -34,6 +46,7 @@ import numpy as np  # noqa: E402
 
 import csparks as cs  # noqa: E402
 import dyad  # noqa: E402
+import hashlib  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
 BYTES_F32 = 1 + 5 * (4 + 4)
@@ -42,6 +55,8 @@ METRIC = "spark realisations/sec (closure vs plain) at KS<0.05; stencil HBM GB/s
 TSTOP_MS = 30.0
 PROFILE_TRAFFIC = os.path.join(REPO, "profiles", "traffic.json")
 WORKLOAD_KEY = "cluster_dyad"
+CLOSURE_PARAMS = os.path.join(REPO, "hpc-ca-phospattern_amd", "closure_params.npz")
+ENSEMBLE_LAUNCH = {"rows": 8, "zchunk": 8, "prefetch": 1, "nontemporal": 0}
 LAUNCH_KEYS = ["rows", "zchunk", "xcd_remap", "prefetch", "nontemporal", "lds_tile", "linear", "yblock",
                "fuse", "fuse_zchunk", "fuse_xcd"]
 
@@ -70,6 +85,113 @@ def cpu_baseline(d, budget_s=15.0):
     return nv * steps / el, cores, steps, el
 
 
+def kernel_source_hash() -> str:
+    """SHA-256 of the HIP sources the library was built from, which the
+    committed traffic profile records: a profile of other sources is stale."""
+    h = hashlib.sha256()
+    csrc = os.path.join(REPO, "hpc-ca-phospattern_amd", "csrc")
+    for name in sorted(os.listdir(csrc)):
+        if name.endswith((".hip", ".h")):
+            with open(os.path.join(csrc, name), "rb") as f:
+                h.update(name.encode() + b"\0" + f.read())
+    return h.hexdigest()
+
+
+def fp64_record(size, steps):
+    """The headline dyad with FP64 fields (one-step passes)."""
+    d = dyad.cluster_dyad(seed=1, n=tuple(size))
+    sim = cs.Simulation(d.geometry, d.model, cs.Options(h=3.0, seed=1, open={"ryr": [0]}, precision="f64"))
+    nv = int(np.prod(sim.n))
+    sim.step(5)
+    sim.synchronize()
+    t1 = time.perf_counter()
+    total_ms, pass_ms, passes, fused = sim.time_passes(steps)
+    sim.synchronize()
+    wall = time.perf_counter() - t1
+    sim.close()
+    ms = wall * 1e3 / steps
+    kern = (steps / passes) * nv * BYTES_F64 / (pass_ms * 1e-3) / 1e9
+    log(f"fp64: {ms:.4f} ms/step, pass {pass_ms * 1e3:.1f} us ({kern:.0f} GB/s)")
+    return {"ms_per_step": round(ms, 4), "value": round(nv * BYTES_F64 / (ms * 1e-3) / 1e9, 1), "unit": "GB/s",
+            "bytes_per_voxel_step": BYTES_F64, "kernel": "k_stencil" if fused == 0 else "k_step2",
+            "kernel_ms": round(pass_ms, 5),
+            "roofline": {"bound": "hbm", "achieved": round(kern, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": round(kern / PEAK_HBM_GBS, 4)}}
+
+
+def ensemble_members(count, h, edge, ryr, seed0, precision, kinetics=None):
+    ds = [dyad.ensemble_dyad(seed0 + k, h=h, edge_nm=edge, ryr=ryr) for k in range(count)]
+    return ds, [(d.geometry, d.model if kinetics is None else kinetics(d.model),
+                 cs.Options(h=h, seed=seed0 + k, open={"ryr": [d.open_index]}, precision=precision))
+                for k, d in enumerate(ds)]
+
+
+def ensemble_rate(ens, steps, tstop_ms, warmup=10):
+    """Measured step time of an ensemble and its realisations/s over runs
+    of tstop_ms."""
+    ens.step(warmup)
+    ens.synchronize()
+    t1 = time.perf_counter()
+    ens.step(steps)
+    ens.synchronize()
+    ms = (time.perf_counter() - t1) * 1e3 / steps
+    run_steps = cs.host_lib().cs_count_steps(tstop_ms, ens.dt)
+    return ms, run_steps, ens.count / (run_steps * ms * 1e-3)
+
+
+def ensemble_record(count=512, steps=100):
+    """Config-2 ensemble: realisations of 256-512 nm dyads at 6 nm."""
+    t0 = time.perf_counter()
+    ds, members = ensemble_members(count, 6.0, (256.0, 512.0), (9, 100), 1, "f32")
+    ens = cs.Ensemble(members, instances=list(range(count)))
+    _, nv, nbytes = ens.info()
+    ens.set_launch(**ENSEMBLE_LAUNCH)
+    log(f"ensemble: {count} dyads, {nv} voxels, set up in {time.perf_counter() - t0:.1f}s")
+    ms, run_steps, rate = ensemble_rate(ens, steps, 20.0)
+    tot_ms, stencil_ms = ens.time_steps(steps)
+    ens.close()
+    gbs = nv * BYTES_F32 / (stencil_ms * 1e-3) / 1e9
+    log(f"ensemble: {ms:.4f} ms/step (stencil {stencil_ms * 1e3:.1f} us, {gbs:.0f} GB/s) -> {rate:.3f} realisations/s")
+    return {"workload": f"{count} dyads, edges 256-512 nm at 6 nm, 9-100 RyRs, 0-50% pRyR, Philox, FP32, 20 ms runs",
+            "realisations": count, "voxels": nv, "ms_per_step": round(ms, 4), "steps_per_realisation": run_steps,
+            "realisations_per_s": round(rate, 4), "launch": ENSEMBLE_LAUNCH,
+            "stencil": {"kernel": "k_batch_stencil", "kernel_ms": round(stencil_ms, 5),
+                        "achieved": round(gbs, 1), "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4)}}
+
+
+def closure_record(count=256, fine_count=16, steps=60, fine_steps=6):
+    """Config-4 comparison: the 12 nm closure ensemble against the plain
+    3 nm ensemble of the same dyads (edges 512-1024 nm, 25-150 RyRs)."""
+    import closure
+    t0 = time.perf_counter()
+    net, norm, trained = closure.load(CLOSURE_PARAMS)
+    ds, members = ensemble_members(count, 12.0, (512.0, 1024.0), (25, 150), 5000, "f32")
+    ens = cs.Ensemble(members, instances=list(range(count)))
+    ens.set_gating(cs.Ensemble.GATE_CLOSURE, net)
+    _, nv12, _ = ens.info()
+    ms12, steps12, rate12 = ensemble_rate(ens, steps, 50.0)
+    flagged = int(np.count_nonzero(ens.monitor()))
+    ens.close()
+    log(f"closure 12 nm: {count} dyads, {nv12} voxels, {ms12:.4f} ms/step -> {rate12:.3f} realisations/s "
+        f"({flagged} flagged by the range monitor)")
+    fine = [(g, m, cs.Options(h=3.0, seed=o.seed, open=o.open, precision="f32")) for g, m, o in members[:fine_count]]
+    ens3 = cs.Ensemble(fine, instances=list(range(fine_count)))
+    _, nv3, _ = ens3.info()
+    ms3, steps3, rate3 = ensemble_rate(ens3, fine_steps, 50.0, warmup=2)
+    ens3.close()
+    net.close()
+    log(f"plain 3 nm: {fine_count} dyads, {nv3} voxels, {ms3:.4f} ms/step -> {rate3:.5f} realisations/s "
+        f"(set up in {time.perf_counter() - t0:.1f}s)")
+    return {"workload": "dyads of 512-1024 nm edges, 25-150 RyRs, 0-50% pRyR, Philox, FP32, 50 ms runs",
+            "closure": {"h_nm": 12.0, "realisations": count, "voxels": nv12, "ms_per_step": round(ms12, 4),
+                        "steps_per_realisation": steps12, "realisations_per_s": round(rate12, 4),
+                        "network": "trained" if trained else "untrained (same cost)",
+                        "range_monitor_flagged": flagged},
+            "plain": {"h_nm": 3.0, "realisations": fine_count, "voxels": nv3, "ms_per_step": round(ms3, 4),
+                      "steps_per_realisation": steps3, "realisations_per_s": round(rate3, 6)},
+            "speedup": round(rate12 / rate3, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +205,7 @@ def main():
                          "default shape, which the committed profiles cover)")
     ap.add_argument("--launch", type=str, default="",
                     help="fixed launch shape " + ",".join(LAUNCH_KEYS) + " (a prefix of them; skips autotune)")
+    ap.add_argument("--no-extra", action="store_true", help="only the headline dyad (no fp64, ensemble, closure)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB (for scaling traffic counters) and exit")
     args = ap.parse_args()
@@ -140,7 +263,7 @@ def main():
             t = json.load(f)
         key = ",".join(str(launch[k]) for k in LAUNCH_KEYS)
         if t.get("workload") == WORKLOAD_KEY and t.get("precision") == args.precision and \
-                tuple(t.get("grid", ())) == tuple(args.size):
+                tuple(t.get("grid", ())) == tuple(args.size) and t.get("source_sha256") == kernel_source_hash():
             traffic = t.get("bytes_per_launch", {}).get(key)
 
     cpu = None
@@ -179,6 +302,14 @@ def main():
         "cpu_baseline": cpu,
         "total_calcium": tot,
     }
+    out["roofline"]["traffic_source"] = ("rocprofv3 FETCH_SIZE/WRITE_SIZE of this launch shape and kernel source "
+                                         "(profiles/traffic.json)" if traffic else "no profile of this kernel source")
+    if traffic:
+        out["roofline"]["hbm_frac"] = round(traffic / (kernel_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)
+    if not args.no_extra:
+        out["fp64"] = fp64_record(args.size, 20)
+        out["ensemble"] = ensemble_record()
+        out["closure"] = closure_record()
     print(json.dumps(out))
 
 

@@ -49,12 +49,26 @@ def _hipcc():
     return path
 
 
+DEVICE_SOURCES = ("csparks.hip", "cs_batch.hip", "cs_closure.hip")
+
+
 def build_device():
+    """Compile the HIP sources to objects side by side, then link them."""
     os.makedirs(OUT, exist_ok=True)
     out = os.path.join(OUT, "libcsparks.so")
     tmp = out + ".tmp"
-    _run([_hipcc(), *HIPCC_FLAGS, "-shared", "-I", INC, "-I", CSRC,
-          os.path.join(CSRC, "csparks.hip"), os.path.join(CSRC, "cs_batch.hip"), "-o", tmp])
+    objs, procs = [], []
+    for src in DEVICE_SOURCES:
+        obj = os.path.join(OUT, os.path.splitext(src)[0] + ".o")
+        objs.append(obj)
+        cmd = [_hipcc(), *HIPCC_FLAGS, "-c", "-I", INC, "-I", CSRC, os.path.join(CSRC, src), "-o", obj]
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+    for cmd, p in procs:
+        log, _ = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(log)
+            raise RuntimeError("build step failed: " + " ".join(cmd))
+    _run([_hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp])
     os.replace(tmp, out)
     return out
 

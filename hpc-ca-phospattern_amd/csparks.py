@@ -301,6 +301,23 @@ def device_lib() -> C.CDLL:
         lib.cs_batch_autotune.argtypes = [P, C.c_uint64, C.c_uint32, C.POINTER(CBatchLaunch), pd]
         lib.cs_batch_set_graph.argtypes = [P, C.c_uint32]
         lib.cs_batch_time_steps.argtypes = [P, C.c_uint64, C.c_uint64, pd, pd]
+        pf = C.POINTER(C.c_float)
+        lib.cs_batch_set_gating.argtypes = [P, C.c_uint32, P]
+        lib.cs_batch_copy_channel_states.argtypes = [P, P]
+        lib.cs_batch_channel_count.argtypes = [P, C.POINTER(C.c_uint32)]
+        lib.cs_batch_closure_features.argtypes = [P, pf]
+        lib.cs_batch_channel_ca.argtypes = [P, pd]
+        lib.cs_batch_closure_predictions.argtypes = [P, pd]
+        lib.cs_batch_monitor.argtypes = [P, C.POINTER(C.c_uint32)]
+        lib.cs_mlp_create.argtypes = [pf, C.POINTER(P)]
+        lib.cs_mlp_destroy.argtypes = [P]
+        lib.cs_mlp_get_params.argtypes = [P, pf]
+        lib.cs_mlp_set_params.argtypes = [P, pf]
+        lib.cs_mlp_set_normalisation.argtypes = [P, pf, pf, pf, pf, C.c_float, C.c_float, C.c_float]
+        lib.cs_mlp_forward.argtypes = [P, pf, C.c_uint64, pf]
+        lib.cs_mlp_grad.argtypes = [P, pf, pf, C.c_uint64, pf, pd]
+        lib.cs_mlp_set_data.argtypes = [P, pf, pf, C.c_uint64, C.c_uint64]
+        lib.cs_mlp_train.argtypes = [P, C.c_uint32, C.c_uint32, C.c_float, pd]
         lib._cs_typed = True
     return lib
 
@@ -1047,6 +1064,57 @@ class Ensemble:
     def set_graph(self, on: bool):
         _check(device_lib().cs_batch_set_graph(self.handle, 1 if on else 0), "cs_batch_set_graph")
 
+    GATE_OWN, GATE_REPLAY, GATE_CLOSURE = 0, 1, 2
+
+    def set_gating(self, mode: int, net: "Optional[Mlp]" = None):
+        """Gate as the reference does (GATE_OWN), not at all (GATE_REPLAY:
+        states come from copy_channel_states), or on the closure network's
+        prediction of each channel mouth's Ca (GATE_CLOSURE)."""
+        self._net = net                       # the network must outlive its use
+        _check(device_lib().cs_batch_set_gating(self.handle, mode, net.handle if net is not None else None),
+               "cs_batch_set_gating")
+
+    def copy_channel_states(self, src: "Ensemble"):
+        _check(device_lib().cs_batch_copy_channel_states(self.handle, src.handle), "cs_batch_copy_channel_states")
+
+    @property
+    def num_channels(self) -> int:
+        n = C.c_uint32()
+        _check(device_lib().cs_batch_channel_count(self.handle, C.byref(n)), "cs_batch_channel_count")
+        return int(n.value)
+
+    def closure_features(self) -> np.ndarray:
+        """Raw closure inputs of every channel, [channels, 30]."""
+        out = np.zeros((max(self.num_channels, 1), MLP_IN), dtype=np.float32)
+        _check(device_lib().cs_batch_closure_features(self.handle, out.ctypes.data_as(C.POINTER(C.c_float))),
+               "cs_batch_closure_features")
+        return out[:self.num_channels]
+
+    def channel_ca(self) -> np.ndarray:
+        """Absolute Ca at every channel's cleft voxel."""
+        out = np.zeros(max(self.num_channels, 1))
+        _check(device_lib().cs_batch_channel_ca(self.handle, out.ctypes.data_as(C.POINTER(C.c_double))),
+               "cs_batch_channel_ca")
+        return out[:self.num_channels]
+
+    def closure_predictions(self) -> np.ndarray:
+        out = np.zeros(max(self.num_channels, 1))
+        _check(device_lib().cs_batch_closure_predictions(self.handle, out.ctypes.data_as(C.POINTER(C.c_double))),
+               "cs_batch_closure_predictions")
+        return out[:self.num_channels]
+
+    def monitor(self) -> np.ndarray:
+        out = np.zeros(self.count, dtype=np.uint32)
+        _check(device_lib().cs_batch_monitor(self.handle, out.ctypes.data_as(C.POINTER(C.c_uint32))),
+               "cs_batch_monitor")
+        return out
+
+    def channel_owner(self) -> np.ndarray:
+        """Realisation of every channel, in channel order."""
+        return np.repeat(np.arange(self.count), [sum(self._flux_size(i, f) for f in range(self.setups[i].problem.num_fluxes)
+                                                     if self.setups[i].problem.fluxes[f].discrete)
+                                                 for i in range(self.count)])
+
     def time_steps(self, nsteps: int):
         tot, ker = C.c_double(), C.c_double()
         _check(device_lib().cs_batch_time_steps(self.handle, self.time_ind, nsteps, C.byref(tot), C.byref(ker)),
@@ -1075,6 +1143,83 @@ class Ensemble:
 
     def __exit__(self, *exc):
         self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+MLP_NUM_PARAMS = 6337
+MLP_IN = 30
+
+
+def _f32(a, shape=None):
+    arr = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None:
+        arr = arr.reshape(shape)
+    return arr, arr.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Mlp:
+    """The closure's network on the GPU (cs_mlp_*): 30-64-64-1 GELU
+    perceptron, bfloat16 matrix-core passes with FP32 accumulation, Adam."""
+
+    def __init__(self, params):
+        arr, ptr = _f32(params)
+        if arr.size != MLP_NUM_PARAMS:
+            raise ValueError(f"expected {MLP_NUM_PARAMS} parameters")
+        self.handle = C.c_void_p()
+        _check(device_lib().cs_mlp_create(ptr, C.byref(self.handle)), "cs_mlp_create")
+
+    @property
+    def params(self) -> np.ndarray:
+        out = np.empty(MLP_NUM_PARAMS, dtype=np.float32)
+        _check(device_lib().cs_mlp_get_params(self.handle, out.ctypes.data_as(C.POINTER(C.c_float))),
+               "cs_mlp_get_params")
+        return out
+
+    @params.setter
+    def params(self, p):
+        arr, ptr = _f32(p)
+        _check(device_lib().cs_mlp_set_params(self.handle, ptr), "cs_mlp_set_params")
+
+    def set_normalisation(self, in_mean, in_std, in_lo, in_hi, out_mean, out_std, margin):
+        a = [_f32(v, (MLP_IN,)) for v in (in_mean, in_std, in_lo, in_hi)]
+        _check(device_lib().cs_mlp_set_normalisation(self.handle, a[0][1], a[1][1], a[2][1], a[3][1], float(out_mean),
+                                                     float(out_std), float(margin)), "cs_mlp_set_normalisation")
+
+    def forward(self, x) -> np.ndarray:
+        xa, xp = _f32(x, (-1, MLP_IN))
+        y = np.empty(len(xa), dtype=np.float32)
+        _check(device_lib().cs_mlp_forward(self.handle, xp, len(xa), y.ctypes.data_as(C.POINTER(C.c_float))),
+               "cs_mlp_forward")
+        return y
+
+    def grad(self, x, t):
+        xa, xp = _f32(x, (-1, MLP_IN))
+        ta, tp = _f32(t, (len(xa),))
+        g = np.empty(MLP_NUM_PARAMS, dtype=np.float32)
+        loss = C.c_double()
+        _check(device_lib().cs_mlp_grad(self.handle, xp, tp, len(xa), g.ctypes.data_as(C.POINTER(C.c_float)),
+                                        C.byref(loss)), "cs_mlp_grad")
+        return g, loss.value
+
+    def set_data(self, x, t, seed: int = 0):
+        xa, xp = _f32(x, (-1, MLP_IN))
+        ta, tp = _f32(t, (len(xa),))
+        _check(device_lib().cs_mlp_set_data(self.handle, xp, tp, len(xa), int(seed)), "cs_mlp_set_data")
+
+    def train(self, steps: int, batch: int, lr: float) -> float:
+        loss = C.c_double()
+        _check(device_lib().cs_mlp_train(self.handle, steps, batch, float(lr), C.byref(loss)), "cs_mlp_train")
+        return loss.value
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            device_lib().cs_mlp_destroy(self.handle)
+            self.handle = C.c_void_p()
 
     def __del__(self):
         try:

@@ -29,6 +29,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cstdlib>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -38,6 +39,7 @@
 
 #include "csparks.h"
 #include "cs_device.h"
+#include "cs_mlp.h"
 
 namespace {
 
@@ -80,6 +82,21 @@ struct BatchChan {
   double rest_to;
   uint32_t flux, ch, state, uni;
 };
+
+/* Where a channel sits, for the closure's features: its realisation, the
+ * grid coordinates of its cleft (to) voxel, and the stored species 0 at
+ * its SR (from) voxel with that voxel's resting value. */
+struct BatchChanGeo {
+  uint32_t inst;
+  int x, y, z;
+  uint64_t elem_from;
+  double rest_from;
+};
+
+/* Neighbour entries of a channel: state slot of a channel of the same
+ * realisation whose cleft voxel lies within Chebyshev distance 2 in the
+ * cleft plane (itself included); bit 31 set when the distance is 2. */
+constexpr uint32_t NB_FAR = 0x80000000u;
 
 /* Step t's fluxes (fluxmask, from u into un) then gating of step gate_ti
  * from un (when gate is set), for every active realisation.  With ti_base
@@ -151,7 +168,7 @@ k_batch_bridge(const T* __restrict__ u, T* __restrict__ un, const DevParams<T>* 
                const BatchInst* __restrict__ insts, const uint8_t* __restrict__ active,
                const BatchChan* __restrict__ chans, const BRow* __restrict__ rows, const BEnt* __restrict__ ents,
                uint8_t* __restrict__ states, uint8_t* __restrict__ recruited, uint64_t* __restrict__ last_open,
-               uint32_t* __restrict__ nopen, BridgeArgs a, const double* __restrict__ uniforms)
+               BridgeArgs a, const double* __restrict__ uniforms, const double* __restrict__ cpred)
 {
   const uint32_t i = blockIdx.x;
   if (active && !active[i]) return;
@@ -185,7 +202,8 @@ k_batch_bridge(const T* __restrict__ u, T* __restrict__ un, const DevParams<T>* 
     const BatchChan b = chans[in.chan_off + c];
     const FluxParams& F = P->flux[b.flux];
     uint8_t sv = states[b.state];
-    const double c0 = F.stochastic ? (double)un[b.elem_to] + b.rest_to : 0.0;
+    /* the cleft voxel's Ca, or the closure's prediction for the mouth */
+    const double c0 = !F.stochastic ? 0.0 : cpred ? cpred[in.chan_off + c] : (double)un[b.elem_to] + b.rest_to;
     const double* draw = (uniforms && b.uni != NO_ROW) ? uniforms + a.uni_base + in.uni_off + b.uni : nullptr;
     sv = gate_one(F, sv, c0, gti, t, dts, in.seed, in.instance, draw, b.ch);
     states[b.state] = sv;
@@ -194,12 +212,119 @@ k_batch_bridge(const T* __restrict__ u, T* __restrict__ un, const DevParams<T>* 
       open++;
     }
   }
-  /* open channels of the realisation after this gating step */
+  /* the latest gating step that left a channel of the realisation open */
   for (int off = 32; off > 0; off >>= 1) open += __shfl_xor(open, off, 64);
-  if (threadIdx.x == 0) {
-    nopen[i] = open;
-    if (open) last_open[i] = gti;
+  if (threadIdx.x == 0 && open) last_open[i] = gti;
+}
+
+/* The closure's raw input k of channel c: the absolute Ca of the 3x3x3
+ * coarse voxels around its cleft voxel (k = 9 (dz+1) + 3 (dy+1) + dx+1,
+ * clamped at the grid faces), the absolute Ca of its SR voxel (27), and
+ * the open channels within Chebyshev distance 1 (28) and 2 (29) in the
+ * cleft plane, itself included; 0 for the padding inputs 30, 31. */
+template <typename T>
+__device__ inline float closure_raw(int k, uint32_t c, const BatchChanGeo& cg, const BatchGeo& gg,
+                                    const T* __restrict__ u, const uint8_t* __restrict__ ids,
+                                    const DevParams<T>* __restrict__ P, const uint8_t* __restrict__ states,
+                                    const uint32_t* __restrict__ nb_start, const uint32_t* __restrict__ nb)
+{
+  if (k < 27) {
+    const int x = min(max(cg.x + k % 3 - 1, 0), gg.nx - 1);
+    const int y = min(max(cg.y + (k / 3) % 3 - 1, 0), gg.ny - 1);
+    const int z = min(max(cg.z + k / 9 - 1, 0), gg.nz - 1);
+    const uint64_t col = (uint64_t)y * gg.nx + x;
+    const int d = ids[gg.id_off + (uint64_t)z * gg.plane + col] & 7;
+    return (float)((double)u[gg.u_off + (uint64_t)z * gg.ns * gg.plane + col] + P->init64[d][0]);
   }
+  if (k == 27) return (float)((double)u[cg.elem_from] + cg.rest_from);
+  if (k == 28 || k == 29) {
+    uint32_t n = 0;
+    for (uint32_t e = nb_start[c]; e < nb_start[c + 1]; e++) {
+      const uint32_t v = nb[e];
+      if (k == 29 || !(v & NB_FAR)) n += states[v & ~NB_FAR];
+    }
+    return (float)n;
+  }
+  return 0.0f;
+}
+
+/* Normalised input: logarithm for the concentrations, then the training
+ * set's standardisation; flags an input outside the monitored range. */
+__device__ inline float closure_input(const MlpDev* __restrict__ N, int k, float raw, bool& out_of_range)
+{
+  if (k >= MLP_IN) return 0.0f;
+  const float f = k < 28 ? logf(fmaxf(raw, 1e-6f)) : raw;
+  const float x = (f - N->in_mean[k]) / N->in_std[k];
+  if (x < N->in_lo[k] - N->margin || x > N->in_hi[k] + N->margin) out_of_range = true;
+  return x;
+}
+
+/* Raw closure inputs of every channel of the ensemble, [channel][30]. */
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_closure_features(const T* __restrict__ u, const uint8_t* __restrict__ ids, const DevParams<T>* __restrict__ P,
+                   const BatchGeo* __restrict__ geos, const BatchChanGeo* __restrict__ cgeo,
+                   const uint8_t* __restrict__ states, const uint32_t* __restrict__ nb_start,
+                   const uint32_t* __restrict__ nb, uint32_t nchan, float* __restrict__ out)
+{
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (uint64_t)nchan * MLP_IN) return;
+  const uint32_t c = (uint32_t)(k / MLP_IN);
+  const BatchChanGeo cg = cgeo[c];
+  out[k] = closure_raw<T>((int)(k % MLP_IN), c, cg, geos[cg.inst], u, ids, P, states, nb_start, nb);
+}
+
+/* Absolute Ca at every channel's cleft voxel (the concentration gating
+ * reads, species.c:2569). */
+template <typename T>
+__global__ void k_channel_ca(const T* __restrict__ u, const BatchChan* __restrict__ chans, uint32_t nchan,
+                             double* __restrict__ out)
+{
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < nchan) out[c] = (double)u[chans[c].elem_to] + chans[c].rest_to;
+}
+
+/* The closure's prediction for every channel: 32 channels per wavefront
+ * (lane l%32; both lane groups gather half of the inputs each), the
+ * network on the matrix cores gives the fine-grid Ca at each channel
+ * mouth, which the bridge then gates on instead of the coarse cleft
+ * voxel's Ca.  A channel whose normalised inputs leave the training range
+ * counts in its realisation's monitor (flags). */
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_closure_predict(const T* __restrict__ u, const uint8_t* __restrict__ ids, const DevParams<T>* __restrict__ P,
+                  const MlpDev* __restrict__ N, const BatchGeo* __restrict__ geos, const uint8_t* __restrict__ active,
+                  const BatchChanGeo* __restrict__ cgeo, const uint8_t* __restrict__ states,
+                  const uint32_t* __restrict__ nb_start, const uint32_t* __restrict__ nb, uint32_t nchan,
+                  double* __restrict__ cpred, uint32_t* __restrict__ flags)
+{
+  const int lane = threadIdx.x & 63, g = lane >> 5;
+  const uint64_t tile = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  if (tile * 32 >= nchan) return;   /* whole wavefront */
+  const uint32_t c = (uint32_t)(tile * 32 + (lane & 31));
+  const bool valid = c < nchan;
+  const uint32_t cc = valid ? c : 0;
+  const BatchChanGeo cg = cgeo[cc];
+  const BatchGeo gg = geos[cg.inst];
+  bool oor = false;
+  bf16x8 x[2];
+#pragma unroll
+  for (int ch = 0; ch < 2; ch++)
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int k = 16 * ch + 8 * g + j;
+      const float raw = valid ? closure_raw<T>(k, cc, cg, gg, u, ids, P, states, nb_start, nb) : 0.0f;
+      x[ch][j] = to_bf16(valid ? closure_input(N, k, raw, oor) : 0.0f);
+    }
+  f32x16 z1[2], z2[2];
+  bf16x8 h1b[2][2];
+  const float y = mlp_forward(N, [&](int seg) { return mlp_opnd(N, seg, lane); }, x, z1, z2, h1b,
+                              [](int, int, float) {});
+  const bool oor_any = oor || __shfl_xor((int)oor, 32, 64);
+  if (g != 0 || !valid) return;
+  if (active && !active[cg.inst]) return;
+  if (oor_any) atomicAdd(&flags[cg.inst], 1u);
+  cpred[c] = exp((double)y * N->out_std + N->out_mean);
 }
 
 /* Output reductions of every realisation: one block per realisation, a
@@ -244,21 +369,26 @@ k_batch_stats(const T* __restrict__ u, const uint8_t* __restrict__ ids, const De
         }
       __syncthreads();
     }
-    /* voxels per domain, summed the same way */
-    for (int d = 0; d < MAXD; d++) sh[0][d][threadIdx.x] = (double)cnt[d];
-    __syncthreads();
-    double n_d = 0.0;
+    /* the block's results, then the voxels per domain through the same
+     * shared array */
+    double rmn = 0.0, rmx = 0.0, rsm = 0.0;
     if (threadIdx.x < MAXD) {
-      for (int k = 0; k < BSTAT_THREADS; k++) n_d += sh[0][threadIdx.x][k];
+      rmn = sh[0][threadIdx.x][0];
+      rmx = sh[1][threadIdx.x][0];
+      rsm = sh[2][threadIdx.x][0];
     }
+    __syncthreads();
+    for (int d = 0; d < MAXD; d++) sh[0][d][threadIdx.x] = (double)cnt[d];
     __syncthreads();
     if (threadIdx.x < MAXD) {
       const int d = threadIdx.x;
+      double n_d = 0.0;
+      for (int k = 0; k < BSTAT_THREADS; k++) n_d += sh[0][d][k];
       const double r = P->init64[d][s];
       double* o = out + (((uint64_t)blockIdx.x * MAXS + s) * MAXD + d) * 3;
-      o[0] = n_d > 0 ? fmin((double)FLT_MAX, sh[0][d][0] + r) : (double)FLT_MAX;
-      o[1] = n_d > 0 ? fmax((double)FLT_MIN, sh[1][d][0] + r) : (double)FLT_MIN;
-      o[2] = sh[2][d][0] + n_d * r * dV;
+      o[0] = n_d > 0 ? fmin((double)FLT_MAX, rmn + r) : (double)FLT_MAX;
+      o[1] = n_d > 0 ? fmax((double)FLT_MIN, rmx + r) : (double)FLT_MIN;
+      o[2] = rsm + n_d * r * dV;
     }
     __syncthreads();
   }
@@ -385,7 +515,17 @@ struct cs_batch {
   uint8_t* d_recruited = nullptr;
   uint8_t* d_active = nullptr;
   uint64_t* d_last_open = nullptr;
-  uint32_t* d_nopen = nullptr;
+  /* closure gating and replay */
+  uint32_t gating = CS_GATE_OWN;
+  const MlpDev* d_net = nullptr;           /* the closure's network            */
+  BatchChanGeo* d_cgeo = nullptr;
+  uint32_t* d_nb_start = nullptr;
+  uint32_t* d_nb = nullptr;
+  double* d_cpred = nullptr;
+  double* d_chca = nullptr;
+  float* d_feat = nullptr;
+  uint32_t* d_flags = nullptr;
+  uint32_t nchan_total = 0;
   double* d_uniforms = nullptr;
   uint64_t uniform_cap = 0;
   uint32_t uni_stride = 0;                 /* stochastic draws per gating step */
@@ -421,7 +561,8 @@ void free_batch(cs_batch* b)
   for (auto g : b->graph_defs) (void)hipGraphDestroy(g);
   void* ptrs[] = {b->d_u[0], b->d_u[1], b->d_ids, b->d_params, b->d_geo, b->d_tiles, b->d_queue, b->d_inst,
                   b->d_chan, b->d_rows, b->d_ents, b->d_states, b->d_recruited, b->d_active, b->d_last_open,
-                  b->d_nopen, b->d_uniforms, b->d_stats, b->d_lw, b->d_lpart, b->d_lproj, b->d_loff, b->d_ti};
+                  b->d_cgeo, b->d_nb_start, b->d_nb, b->d_cpred, b->d_chca, b->d_feat, b->d_flags,
+                  b->d_uniforms, b->d_stats, b->d_lw, b->d_lpart, b->d_lproj, b->d_loff, b->d_ti};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -455,16 +596,21 @@ int same_model(const cs_problem& a, const cs_problem& p, uint32_t i)
   return CS_OK;
 }
 
-/* Column tiles of every realisation: `cols` columns by zchunk planes. */
+/* Column tiles of every realisation: `cols` columns by at most zchunk
+ * planes, a realisation's planes split into chunks whose sizes differ by
+ * at most one (25 planes by 8: 7, 6, 6, 6 rather than 8, 8, 8, 1, since
+ * a chunk also reads the planes either side of it). */
 void make_tiles(const cs_batch* b, uint32_t cols, uint32_t zchunk, std::vector<BatchTile>& out)
 {
   out.clear();
   for (uint32_t i = 0; i < b->count; i++) {
     const BatchGeo& g = b->geo[i];
-    const uint32_t zc = std::max<uint32_t>(1, std::min<uint32_t>(zchunk, (uint32_t)g.nz));
-    for (uint32_t z0 = 0; z0 < (uint32_t)g.nz; z0 += zc)
-      for (uint64_t c = 0; c < g.plane; c += cols)
-        out.push_back(BatchTile{i, (uint32_t)c, z0, std::min<uint32_t>((uint32_t)g.nz, z0 + zc)});
+    const uint32_t nz = (uint32_t)g.nz;
+    const uint32_t k = (nz + std::max<uint32_t>(1, zchunk) - 1) / std::max<uint32_t>(1, zchunk);
+    for (uint32_t q = 0; q < k; q++) {
+      const uint32_t z0 = q * nz / k, z1 = (q + 1) * nz / k;
+      for (uint64_t c = 0; c < g.plane; c += cols) out.push_back(BatchTile{i, (uint32_t)c, z0, z1});
+    }
   }
 }
 
@@ -552,11 +698,12 @@ uint32_t b_fluxmask(const cs_batch* b, uint64_t ti)
 bool b_gates(const cs_batch* b, uint64_t ti) { return b->ndiscrete && ti % b->model.stochastic_substep == 0; }
 
 template <typename T>
-void launch_bbridge(cs_batch* b, const void* u, void* un, const BridgeArgs& a, const double* d_uni)
+void launch_bbridge(cs_batch* b, const void* u, void* un, const BridgeArgs& a, const double* d_uni,
+                    const double* cpred = nullptr)
 {
   hipLaunchKernelGGL(k_batch_bridge<T>, dim3(b->count), dim3(BRIDGE_BATCH_THREADS), 0, b->stream, (const T*)u,
                      (T*)un, (const DevParams<T>*)b->d_params, b->d_inst, b->d_active, b->d_chan, b->d_rows,
-                     b->d_ents, b->d_states, b->d_recruited, b->d_last_open, b->d_nopen, a, d_uni);
+                     b->d_ents, b->d_states, b->d_recruited, b->d_last_open, a, d_uni, cpred);
 }
 
 /* Steps [first, first + n) on the stream: gating of the first step, then
@@ -573,12 +720,29 @@ int b_run(cs_batch* b, uint64_t first, uint64_t n, const double* d_uni, hipEvent
   auto bridge = [&](uint32_t fluxmask, uint64_t gti, bool gate_next, const void* u, void* un) {
     BridgeArgs a{};
     a.fluxmask = fluxmask;
-    a.gate = (gate_next && b_gates(b, gti)) ? 1u : 0u;
+    a.gate = (gate_next && b->gating != CS_GATE_REPLAY && b_gates(b, gti)) ? 1u : 0u;
     a.gate_ti = d_ti ? gti - first : gti;
     a.ti_base = d_ti;
     a.uni_base = d_uni ? gate_idx * b->uni_stride : 0;
     a.D = D;
     if (a.gate) gate_idx++;
+    if (b->gating == CS_GATE_CLOSURE && a.gate) {
+      /* fluxes first, then the closure's prediction from the new state,
+       * then gating on it */
+      if (a.fluxmask) {
+        BridgeArgs f = a;
+        f.gate = 0;
+        launch_bbridge<T>(b, u, un, f, d_uni);
+      }
+      const unsigned blocks = (b->nchan_total + 127) / 128;
+      if (blocks)
+        hipLaunchKernelGGL(k_closure_predict<T>, dim3(blocks), dim3(256), 0, b->stream, (const T*)un, b->d_ids,
+                           (const DevParams<T>*)b->d_params, b->d_net, b->d_geo, b->d_active, b->d_cgeo, b->d_states,
+                           b->d_nb_start, b->d_nb, b->nchan_total, b->d_cpred, b->d_flags);
+      a.fluxmask = 0;
+      launch_bbridge<T>(b, un, un, a, d_uni, b->d_cpred);
+      return;
+    }
     if (a.fluxmask || a.gate) launch_bbridge<T>(b, u, un, a, d_uni);
   };
   if (n == 0) return CS_OK;
@@ -640,8 +804,12 @@ int b_step(cs_batch* b, uint64_t first, uint64_t n, const double* host_uni, uint
       HIP_TRY(hipMalloc(&b->d_uniforms, sizeof(double) * need));
       b->uniform_cap = need;
     }
-    if (need)
+    if (need) {
+      /* the caller's array may be freed once this call returns, and a copy
+       * from pageable memory can still be reading it then: wait for it */
       HIP_TRY(hipMemcpyAsync(b->d_uniforms, host_uni, sizeof(double) * need, hipMemcpyHostToDevice, b->stream));
+      HIP_TRY(hipStreamSynchronize(b->stream));
+    }
     d_uni = b->d_uniforms;
     return b_run<T>(b, first, n, d_uni, nullptr);
   }
@@ -761,6 +929,8 @@ int cs_batch_create(const cs_problem* problems, uint32_t count, cs_batch** out)
   std::vector<uint8_t> ids_all;
   std::vector<BatchInst> insts(count);
   std::vector<BatchChan> chans;
+  std::vector<BatchChanGeo> cgeo;
+  std::vector<uint32_t> nbs{0}, nbl;
   std::vector<BRow> rows;
   std::vector<BEnt> ents;
   b->geo.resize(count);
@@ -863,13 +1033,24 @@ int cs_batch_create(const cs_problem* problems, uint32_t count, cs_batch** out)
       b->flux_state_off[i][f] = (uint32_t)b->init_states.size();
       const bool stochastic = !(F.params[8] > 0);
       for (uint32_t k = 0; k < F.num_pairs; k++) {
-        const uint32_t to = pairs_dev[2 * (pair_off[f] + k) + 1];
+        const uint32_t from = pairs_dev[2 * (pair_off[f] + k)], to = pairs_dev[2 * (pair_off[f] + k) + 1];
         const uint32_t st = (uint32_t)b->init_states.size();
         chans.push_back(BatchChan{elem_of(0, to), p.init[dom(to)][0], f, k, st, stochastic ? ndraw++ : NO_ROW});
+        cgeo.push_back(BatchChanGeo{i, (int)(to % n0), (int)((to / n0) % n1), (int)(to / g.plane), elem_of(0, from),
+                                    p.init[dom(from)][0]});
         b->init_states.push_back(F.init_states[k] ? 1 : 0);
       }
     }
     in.nchan = (uint32_t)chans.size() - in.chan_off;
+    /* neighbours of each channel within Chebyshev distance 2 */
+    for (uint32_t a = in.chan_off; a < in.chan_off + in.nchan; a++) {
+      for (uint32_t c2 = in.chan_off; c2 < in.chan_off + in.nchan; c2++) {
+        const int d = std::max({std::abs(cgeo[a].x - cgeo[c2].x), std::abs(cgeo[a].y - cgeo[c2].y),
+                                std::abs(cgeo[a].z - cgeo[c2].z)});
+        if (d <= 2) nbl.push_back(chans[c2].state | (d == 2 ? NB_FAR : 0u));
+      }
+      nbs.push_back((uint32_t)nbl.size());
+    }
     b->nchan[i] = in.nchan;
     b->stoch[i] = ndraw;
     b->uni_stride += ndraw;
@@ -896,6 +1077,7 @@ int cs_batch_create(const cs_problem* problems, uint32_t count, cs_batch** out)
   }
   b->total_elems = uoff;
   b->total_ids = idoff;
+  b->nchan_total = (uint32_t)chans.size();
 
   auto upload = [&](void** dst, const void* src, size_t bytes) -> int {
     HIP_TRY(hipMalloc(dst, std::max<size_t>(bytes, 16)));
@@ -913,6 +1095,9 @@ int cs_batch_create(const cs_problem* problems, uint32_t count, cs_batch** out)
   if ((rc = upload((void**)&b->d_geo, b->geo.data(), sizeof(BatchGeo) * count)) != CS_OK ||
       (rc = upload((void**)&b->d_inst, insts.data(), sizeof(BatchInst) * count)) != CS_OK ||
       (rc = upload((void**)&b->d_chan, chans.data(), sizeof(BatchChan) * chans.size())) != CS_OK ||
+      (rc = upload((void**)&b->d_cgeo, cgeo.data(), sizeof(BatchChanGeo) * cgeo.size())) != CS_OK ||
+      (rc = upload((void**)&b->d_nb_start, nbs.data(), sizeof(uint32_t) * nbs.size())) != CS_OK ||
+      (rc = upload((void**)&b->d_nb, nbl.data(), sizeof(uint32_t) * nbl.size())) != CS_OK ||
       (rc = upload((void**)&b->d_rows, rows.data(), sizeof(BRow) * rows.size())) != CS_OK ||
       (rc = upload((void**)&b->d_ents, ents.data(), sizeof(BEnt) * ents.size())) != CS_OK ||
       (rc = upload((void**)&b->d_states, b->init_states.data(), b->init_states.size())) != CS_OK ||
@@ -921,7 +1106,8 @@ int cs_batch_create(const cs_problem* problems, uint32_t count, cs_batch** out)
   std::vector<uint8_t> act(count, 1);
   if ((rc = upload((void**)&b->d_active, act.data(), count)) != CS_OK) return bail(rc);
   if (hipMalloc(&b->d_last_open, sizeof(uint64_t) * count) != hipSuccess ||
-      hipMalloc(&b->d_nopen, sizeof(uint32_t) * count) != hipSuccess ||
+      hipMalloc(&b->d_flags, sizeof(uint32_t) * count) != hipSuccess ||
+      hipMalloc(&b->d_cpred, sizeof(double) * std::max<size_t>(chans.size(), 1)) != hipSuccess ||
       hipMalloc(&b->d_queue, sizeof(uint32_t) * (NQUEUE + 1)) != hipSuccess ||
       hipMalloc(&b->d_ti, sizeof(uint64_t)) != hipSuccess ||
       hipMalloc(&b->d_stats, sizeof(double) * count * MAXS * MAXD * 3) != hipSuccess)
@@ -934,11 +1120,13 @@ int cs_batch_create(const cs_problem* problems, uint32_t count, cs_batch** out)
     fill_params<double>(dp, &problems[0], b->ndiscrete);
     b->rfast = dp.rfast;
     if ((rc = upload(&b->d_params, &dp, sizeof dp)) != CS_OK) return bail(rc);
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return bail(set_err(CS_EDEVICE, "upload failed"));
   } else {
     DevParams<float> dp;
     fill_params<float>(dp, &problems[0], b->ndiscrete);
     b->rfast = dp.rfast;
     if ((rc = upload(&b->d_params, &dp, sizeof dp)) != CS_OK) return bail(rc);
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return bail(set_err(CS_EDEVICE, "upload failed"));
   }
   rc = b->esize == 8 ? upload_tiles<double>(b) : upload_tiles<float>(b);
   if (rc != CS_OK) return bail(rc);
@@ -978,11 +1166,9 @@ int cs_batch_reset(cs_batch* b)
                            b->stream));
   }
   std::vector<uint64_t> lo(b->count, 0);
-  std::vector<uint32_t> no(b->count, 0);
-  for (uint32_t i = 0; i < b->count; i++)
-    for (uint32_t c = 0; c < b->nchan[i]; c++) no[i] += b->init_states[b->state_off[i] + c];
+  std::vector<uint32_t> fl(b->count, 0);
   HIP_TRY(hipMemcpyAsync(b->d_last_open, lo.data(), sizeof(uint64_t) * b->count, hipMemcpyHostToDevice, b->stream));
-  HIP_TRY(hipMemcpyAsync(b->d_nopen, no.data(), sizeof(uint32_t) * b->count, hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipMemcpyAsync(b->d_flags, fl.data(), sizeof(uint32_t) * b->count, hipMemcpyHostToDevice, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   return CS_OK;
 }
@@ -1196,21 +1382,25 @@ int cs_batch_channel_summary(cs_batch* b, uint32_t* nopen, uint64_t* last_open, 
 {
   if (!b) return set_err(CS_EINVAL, "ensemble is NULL");
   HIP_TRY(hipSetDevice(b->device));
-  if (nopen) HIP_TRY(hipMemcpyAsync(nopen, b->d_nopen, sizeof(uint32_t) * b->count, hipMemcpyDeviceToHost, b->stream));
   if (last_open)
     HIP_TRY(hipMemcpyAsync(last_open, b->d_last_open, sizeof(uint64_t) * b->count, hipMemcpyDeviceToHost, b->stream));
-  std::vector<uint8_t> rec;
-  if (nrecruited && !b->init_states.empty()) {
+  std::vector<uint8_t> rec, st;
+  if (!b->init_states.empty()) {
     rec.resize(b->init_states.size());
-    HIP_TRY(hipMemcpyAsync(rec.data(), b->d_recruited, rec.size(), hipMemcpyDeviceToHost, b->stream));
+    st.resize(b->init_states.size());
+    if (nrecruited) HIP_TRY(hipMemcpyAsync(rec.data(), b->d_recruited, rec.size(), hipMemcpyDeviceToHost, b->stream));
+    if (nopen) HIP_TRY(hipMemcpyAsync(st.data(), b->d_states, st.size(), hipMemcpyDeviceToHost, b->stream));
   }
   HIP_TRY(hipStreamSynchronize(b->stream));
-  if (nrecruited)
-    for (uint32_t i = 0; i < b->count; i++) {
-      uint32_t k = 0;
-      for (uint32_t c = 0; c < b->nchan[i]; c++) k += rec[b->state_off[i] + c];
-      nrecruited[i] = k;
+  for (uint32_t i = 0; i < b->count; i++) {
+    uint32_t k = 0, o = 0;
+    for (uint32_t c = 0; c < b->nchan[i]; c++) {
+      if (nrecruited) k += rec[b->state_off[i] + c];
+      if (nopen) o += st[b->state_off[i] + c];
     }
+    if (nrecruited) nrecruited[i] = k;
+    if (nopen) nopen[i] = o;
+  }
   return CS_OK;
 }
 
@@ -1304,6 +1494,105 @@ int cs_batch_time_steps(cs_batch* b, uint64_t first, uint64_t n, double* total_m
   } else {
     *stencil_ms = 0.0;
   }
+  return CS_OK;
+}
+
+int cs_batch_set_gating(cs_batch* b, uint32_t mode, const cs_mlp* net)
+{
+  if (!b) return set_err(CS_EINVAL, "ensemble is NULL");
+  if (mode > CS_GATE_CLOSURE) return set_err(CS_EINVAL, "unknown gating mode %u", mode);
+  if (mode == CS_GATE_CLOSURE) {
+    if (!net) return set_err(CS_EINVAL, "closure gating needs a network");
+    if (b->model.rng == CS_RNG_MT && b->uni_stride) return set_err(CS_EINVAL, "closure gating draws Philox uniforms");
+    b->d_net = (const MlpDev*)cs_internal_mlp_device(net);
+  }
+  b->gating = mode;
+  drop_graphs(b);
+  return CS_OK;
+}
+
+int cs_batch_copy_channel_states(cs_batch* dst, const cs_batch* src)
+{
+  if (!dst || !src) return set_err(CS_EINVAL, "NULL argument");
+  if (dst->count != src->count || dst->nchan != src->nchan)
+    return set_err(CS_EINVAL, "the ensembles' realisations have different channels");
+  if (dst->device != src->device) return set_err(CS_EINVAL, "the ensembles live on different devices");
+  HIP_TRY(hipSetDevice(dst->device));
+  const size_t n = dst->init_states.size();
+  if (!n) return CS_OK;
+  /* the source's pending steps first, then the copies on dst's stream */
+  HIP_TRY(hipStreamSynchronize(src->stream));
+  HIP_TRY(hipMemcpyAsync(dst->d_states, src->d_states, n, hipMemcpyDeviceToDevice, dst->stream));
+  HIP_TRY(hipMemcpyAsync(dst->d_recruited, src->d_recruited, n, hipMemcpyDeviceToDevice, dst->stream));
+  HIP_TRY(hipMemcpyAsync(dst->d_last_open, src->d_last_open, sizeof(uint64_t) * dst->count, hipMemcpyDeviceToDevice,
+                         dst->stream));
+  return CS_OK;
+}
+
+int cs_batch_channel_count(const cs_batch* b, uint32_t* nchan)
+{
+  if (!b || !nchan) return set_err(CS_EINVAL, "NULL argument");
+  *nchan = b->nchan_total;
+  return CS_OK;
+}
+
+int cs_batch_closure_features(cs_batch* b, float* out)
+{
+  if (!b || !out) return set_err(CS_EINVAL, "NULL argument");
+  HIP_TRY(hipSetDevice(b->device));
+  const uint32_t n = b->nchan_total;
+  if (!n) return CS_OK;
+  if (!b->d_feat) HIP_TRY(hipMalloc(&b->d_feat, sizeof(float) * (size_t)n * MLP_IN));
+  const unsigned blocks = (unsigned)(((uint64_t)n * MLP_IN + 255) / 256);
+  if (b->esize == 8)
+    hipLaunchKernelGGL(k_closure_features<double>, dim3(blocks), dim3(256), 0, b->stream, (const double*)b->d_u[b->cur],
+                       b->d_ids, (const DevParams<double>*)b->d_params, b->d_geo, b->d_cgeo, b->d_states,
+                       b->d_nb_start, b->d_nb, n, b->d_feat);
+  else
+    hipLaunchKernelGGL(k_closure_features<float>, dim3(blocks), dim3(256), 0, b->stream, (const float*)b->d_u[b->cur],
+                       b->d_ids, (const DevParams<float>*)b->d_params, b->d_geo, b->d_cgeo, b->d_states,
+                       b->d_nb_start, b->d_nb, n, b->d_feat);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, b->d_feat, sizeof(float) * (size_t)n * MLP_IN, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return CS_OK;
+}
+
+int cs_batch_channel_ca(cs_batch* b, double* out)
+{
+  if (!b || !out) return set_err(CS_EINVAL, "NULL argument");
+  HIP_TRY(hipSetDevice(b->device));
+  const uint32_t n = b->nchan_total;
+  if (!n) return CS_OK;
+  if (!b->d_chca) HIP_TRY(hipMalloc(&b->d_chca, sizeof(double) * n));
+  if (b->esize == 8)
+    hipLaunchKernelGGL(k_channel_ca<double>, dim3((n + 255) / 256), dim3(256), 0, b->stream,
+                       (const double*)b->d_u[b->cur], b->d_chan, n, b->d_chca);
+  else
+    hipLaunchKernelGGL(k_channel_ca<float>, dim3((n + 255) / 256), dim3(256), 0, b->stream,
+                       (const float*)b->d_u[b->cur], b->d_chan, n, b->d_chca);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, b->d_chca, sizeof(double) * n, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return CS_OK;
+}
+
+int cs_batch_closure_predictions(cs_batch* b, double* out)
+{
+  if (!b || !out) return set_err(CS_EINVAL, "NULL argument");
+  HIP_TRY(hipSetDevice(b->device));
+  if (b->nchan_total)
+    HIP_TRY(hipMemcpyAsync(out, b->d_cpred, sizeof(double) * b->nchan_total, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return CS_OK;
+}
+
+int cs_batch_monitor(cs_batch* b, uint32_t* flags)
+{
+  if (!b || !flags) return set_err(CS_EINVAL, "NULL argument");
+  HIP_TRY(hipSetDevice(b->device));
+  HIP_TRY(hipMemcpyAsync(flags, b->d_flags, sizeof(uint32_t) * b->count, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
   return CS_OK;
 }
 

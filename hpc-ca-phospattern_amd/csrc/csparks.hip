@@ -1466,6 +1466,7 @@ int upload_params(cs_context* c)
   c->rfast = dp.rfast;
   if (!c->d_params) HIP_TRY(hipMalloc(&c->d_params, sizeof(DevParams<T>)));
   HIP_TRY(hipMemcpyAsync(c->d_params, &dp, sizeof dp, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));   /* dp lives on this stack frame */
   return CS_OK;
 }
 
@@ -1718,9 +1719,13 @@ int run_steps(cs_context* c, uint64_t first, uint64_t n, const double* host_uni,
       HIP_TRY(hipMalloc(&c->d_uniforms, sizeof(double) * need));
       c->uniform_cap = need;
     }
-    if (need)
+    if (need) {
+      /* the caller's array may be freed once this call returns, and a copy
+       * from pageable memory can still be reading it then: wait for it */
       HIP_TRY(hipMemcpyAsync(c->d_uniforms, host_uni, sizeof(double) * need, hipMemcpyHostToDevice,
                              c->stream));
+      HIP_TRY(hipStreamSynchronize(c->stream));
+    }
     d_uni = c->d_uniforms;
   } else if (host_uni && count) {
     return set_err(CS_EINVAL, "this context uses Philox gating; no uniforms expected");

@@ -234,14 +234,50 @@ def fast_spark_dyad(seed: int = 3, num_ryr: int = 16, num_pryr: int = 4) -> Dyad
     return Dyad(d.geometry, model, d.open_index, f"fast-gating {d.description}")
 
 
+def _cluster(rng: np.random.Generator, count: int, radius_vox: float, h: float,
+             center: Tuple[int, int]) -> List[Tuple[int, int]]:
+    """`count` distinct voxels in a disk by random sequential adsorption:
+    candidates are drawn uniformly in the disk and kept when no kept point
+    lies closer than the spacing.  The spacing starts at 25 nm, or lower
+    when the disk cannot hold `count` points at that spacing (about 0.55
+    of the close packing of disks), and shrinks by 10% whenever 40 draws
+    per point have not placed them all."""
+    cx, cy = center
+    area = math.pi * radius_vox ** 2
+    spacing = min(25.0 / h, 0.9 * math.sqrt(0.55 * area / (math.pi / 4) / count))
+    cells = {(x, y) for x in range(int(cx - radius_vox) - 1, int(cx + radius_vox) + 2)
+             for y in range(int(cy - radius_vox) - 1, int(cy + radius_vox) + 2)
+             if (x - cx) ** 2 + (y - cy) ** 2 <= radius_vox ** 2}
+    if len(cells) < count:
+        raise RuntimeError("the cluster disk has fewer voxels than channels")
+    while True:
+        d2 = max(spacing, 1.0) ** 2
+        pts: List[Tuple[int, int]] = []
+        arr = np.empty((count, 2))
+        r = radius_vox * np.sqrt(rng.random(40 * count))
+        a = 2 * math.pi * rng.random(40 * count)
+        xs = np.rint(cx + r * np.cos(a)).astype(int)
+        ys = np.rint(cy + r * np.sin(a)).astype(int)
+        for x, y in zip(xs, ys):
+            if (x, y) not in cells:
+                continue
+            k = len(pts)
+            if k and np.min((arr[:k, 0] - x) ** 2 + (arr[:k, 1] - y) ** 2) < d2:
+                continue
+            arr[k] = (x, y)
+            pts.append((int(x), int(y)))
+            if len(pts) == count:
+                return pts
+        spacing *= 0.9
+
+
 def ensemble_dyad(seed: int, h: float = 6.0, edge_nm: Tuple[float, float] = (256.0, 512.0),
                   ryr: Tuple[int, int] = (9, 100), pryr_max: float = 0.5, depth_nm: float = 150.0) -> Dyad:
     """One realisation of the ensemble workload, drawn from `seed`: a square
     dyad of edge L ~ U[edge_nm] and depth depth_nm at voxel size h, with
     the lattice dyad's layers (T-tubule 30 nm, cleft 12 nm, jSR 30 nm); a
-    cluster of N ~ U{ryr} RyRs by Poisson-disk sampling in a disk sized to
-    about one RyR per (27 nm)^2, the spacing relaxed from 25 nm until all
-    fit; a fraction ~ U[0, pryr_max] of them phosphorylated (the p_ryr
+    cluster of N ~ U{ryr} RyRs by Poisson-disk sampling (_cluster) in a
+    disk sized to about one RyR per (27 nm)^2; a fraction ~ U[0, pryr_max] of them phosphorylated (the p_ryr
     boundary exists even when empty, so every realisation has the same
     fluxes); the jSR/cleft footprint covers the cluster and at least 62.5%
     of the edge.  open_index is the non-phosphorylated RyR nearest the
@@ -257,16 +293,7 @@ def ensemble_dyad(seed: int, h: float = 6.0, edge_nm: Tuple[float, float] = (256
         raise ValueError("dyad too shallow for its layers")
     c = nxy // 2
     radius_nm = min(27.0 * math.sqrt(count / math.pi) * 1.15, 0.5 * (nxy - 4) * h)
-    pts = None
-    for spacing in (25.0, 22.0, 19.0, 16.0, 13.0, 10.0, 7.0, h):
-        try:
-            pts = poisson_disk(rng, count, radius_vox=radius_nm / h, min_dist_vox=max(1.0, spacing / h),
-                               center=(c, c), max_tries=4000)
-            break
-        except RuntimeError:
-            continue
-    if pts is None:
-        raise RuntimeError("cannot place the cluster")
+    pts = _cluster(rng, count, radius_nm / h, h, (c, c))
     half = max(int(math.ceil(radius_nm / h)) + 2, int(round(0.625 * nxy / 2)))
     half = min(half, c - 1, nxy - c - 1)
     lay = DyadLayout(n=(nxy, nxy, nz), h=h, tt=tt, cleft=cleft, jsr=jsr,

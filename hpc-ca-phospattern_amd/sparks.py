@@ -117,3 +117,22 @@ def realise_batch(runs, nsteps: int, save_every: int, species: int, offsets: Seq
         rec = sum(int(np.sum(r.recruited(k))) for k in range(discrete_count(r)))
         out.append(SparkStats(peak=float(p.max()), duration=fdhm(np.asarray(times), p), recruited=rec))
     return out
+
+
+def realise_ensemble(ens, nsteps: int, save_every: int, species: int, offsets: Sequence[int],
+                     conv: Sequence[float], axis: int = 0, num_domains: int = 0):
+    """realise() for every member of a csparks.Ensemble: the members step
+    together and one batched linescan per output serves them all."""
+    f0 = ens.linescans(species, axis, offsets, conv, num_domains)
+    times, peaks = [0.0], [[0.0] for _ in range(ens.count)]
+    done = 0
+    while done < nsteps:
+        k = min(save_every, nsteps - done)
+        ens.step(k)
+        done += k
+        times.append(done * ens.dt)
+        for i, f in enumerate(ens.linescans(species, axis, offsets, conv, num_domains)):
+            peaks[i].append(float(np.max((f - f0[i]) / f0[i])))
+    _, _, recruited = ens.summary()
+    return [SparkStats(peak=float(np.max(p)), duration=fdhm(np.asarray(times), np.asarray(p)),
+                       recruited=int(r)) for p, r in zip(peaks, recruited)]

@@ -1,0 +1,57 @@
+#!/usr/bin/env python3
+"""Train the closure network on paired 3 nm / 12 nm runs of generated
+12 nm dyads and write its parameters (closure.save).
+
+    python train_closure.py [out.npz] [--dyads N] [--coarse-steps K] [--steps S]
+
+The fine ensemble gates with Ca-independent rates (closure.schedule_model)
+so that the samples cover many open/closed configurations; the coarse
+ensemble replays its channel states.  Held-out dyads give the log-Ca error
+of the closure and of the plain coarse grid, printed at the end."""
+import argparse
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+import numpy as np  # noqa: E402
+
+import closure  # noqa: E402
+import dyad  # noqa: E402
+import mlp_init  # noqa: E402
+
+
+def dyads(seeds, edge, ryr):
+    return [dyad.ensemble_dyad(s, h=12.0, edge_nm=edge, ryr=ryr) for s in seeds]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("out", nargs="?", default=os.path.join(HERE, "closure_params.npz"))
+    ap.add_argument("--dyads", type=int, default=48)
+    ap.add_argument("--coarse-steps", type=int, default=600)
+    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--edge", type=float, nargs=2, default=(256.0, 1024.0))
+    ap.add_argument("--ryr", type=int, nargs=2, default=(9, 150))
+    args = ap.parse_args()
+    t0 = time.time()
+    train = closure.paired_samples(dyads(range(args.dyads), args.edge, args.ryr), list(range(args.dyads)),
+                                   coarse_steps=args.coarse_steps, sample_every=2)
+    print(f"{len(train.ca_fine)} training samples from {args.dyads} dyads "
+          f"({train.fine_steps} fine / {train.coarse_steps} coarse steps) in {time.time() - t0:.1f}s", flush=True)
+    t1 = time.time()
+    net, norm = closure.train(train, mlp_init.params(0), steps=args.steps, batch=2048, lr=2e-3)
+    print(f"trained {args.steps} Adam steps in {time.time() - t1:.1f}s", flush=True)
+    held = closure.paired_samples(dyads(range(10000, 10012), args.edge, args.ryr), list(range(10000, 10012)),
+                                  coarse_steps=args.coarse_steps // 2, sample_every=4)
+    e_c, e_p = closure.log_errors(net, norm, held)
+    print(f"held-out log-Ca MSE at the channel mouth: closure {e_c:.5f}, plain coarse grid {e_p:.5f}", flush=True)
+    closure.save(args.out, net, norm, held_out_mse=np.array([e_c, e_p]),
+                 samples=np.array([len(train.ca_fine)]))
+    print(f"wrote {args.out}", flush=True)
+
+
+if __name__ == "__main__":
+    main()

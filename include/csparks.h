@@ -296,6 +296,35 @@ int cs_batch_channel_summary(cs_batch* b, uint32_t* nopen, uint64_t* last_open_t
  * keep their state.  The stencil pass drops their tiles. */
 int cs_batch_set_active(cs_batch* b, const uint8_t* active);
 
+/* Where an ensemble's gating takes its channel states from:
+ *   CS_GATE_OWN      gating as cs_step does (the default);
+ *   CS_GATE_REPLAY   no gating: the states change only by
+ *                    cs_batch_copy_channel_states (schedule replay);
+ *   CS_GATE_CLOSURE  gating as cs_step does, on the concentration the
+ *                    closure network predicts for each channel mouth
+ *                    instead of the cleft voxel's (Philox only).
+ * The network must outlive the ensemble's use of it.  No reference
+ * counterpart: the reference always gates on the cleft voxel. */
+enum { CS_GATE_OWN = 0, CS_GATE_REPLAY = 1, CS_GATE_CLOSURE = 2 };
+struct cs_mlp;
+int cs_batch_set_gating(cs_batch* b, uint32_t mode, const struct cs_mlp* net);
+/* dst's channel states := src's (same realisations and channels). */
+int cs_batch_copy_channel_states(cs_batch* dst, const cs_batch* src);
+/* Channels over all realisations, in realisation order. */
+int cs_batch_channel_count(const cs_batch* b, uint32_t* nchan);
+/* The closure's raw inputs for every channel, [nchan][30]: absolute Ca
+ * of the 3x3x3 voxels around its cleft voxel (index 9 (dz+1) + 3 (dy+1)
+ * + dx+1, clamped at the faces), absolute Ca of its SR voxel, open
+ * channels within Chebyshev distance 1 and 2 (itself included). */
+int cs_batch_closure_features(cs_batch* b, float* features);
+/* Absolute Ca at every channel's cleft voxel, [nchan]. */
+int cs_batch_channel_ca(cs_batch* b, double* ca);
+/* The closure's latest prediction for every channel mouth, [nchan]. */
+int cs_batch_closure_predictions(cs_batch* b, double* ca);
+/* Per realisation: channel evaluations whose normalised closure inputs
+ * left the training range since cs_batch_reset (the range monitor). */
+int cs_batch_monitor(cs_batch* b, uint32_t* flags);
+
 /* Launch shape of the ensemble's stencil pass: blocks of 64 x rows
  * threads take tiles of 64*rows consecutive plane columns of one
  * realisation by zchunk planes from eight per-XCD work queues; prefetch
@@ -316,6 +345,49 @@ int cs_batch_set_graph(cs_batch* b, uint32_t on);
 /* Time nsteps steps: total ms, and the mean stencil pass (0 with graphs). */
 int cs_batch_time_steps(cs_batch* b, uint64_t first_time_ind, uint64_t nsteps, double* total_ms,
                         double* stencil_ms);
+
+/* ---- learned closure ----------------------------------------------------
+ *
+ * A sub-grid closure for the coarse-grid method, with no reference
+ * counterpart: the reference gates each channel on the concentration of
+ * its cleft voxel (evaluate_RyR_stochastically, boundaryfluxes.c:295-359),
+ * which at 12 nm dilutes the nanodomain a channel sees at 3 nm.  A
+ * 30-64-64-1 perceptron with GELU units predicts the fine-grid calcium at
+ * the channel mouth from coarse-grid features; it runs on the matrix
+ * cores in bfloat16 with FP32 accumulation.
+ *
+ * Parameters are packed FP32 (CS_MLP_NUM_PARAMS): W1 [32][64] (row =
+ * input; rows 30, 31 multiply zero padding), b1 [64], W2 [64][64] (row =
+ * input), b2 [64], w3 [64], b3 [1].  Inputs x [n][30] are normalised
+ * features; the output y is the normalised logarithm of the
+ * concentration (cs_mlp_set_normalisation). */
+#define CS_MLP_NUM_PARAMS 6337
+typedef struct cs_mlp cs_mlp;   /* declared with the ensembles above */
+
+int cs_mlp_create(const float* params, cs_mlp** out);
+int cs_mlp_destroy(cs_mlp* m);
+int cs_mlp_num_params(void);
+int cs_mlp_get_params(cs_mlp* m, float* params);
+int cs_mlp_set_params(cs_mlp* m, const float* params);
+/* Feature normalisation the ensemble applies before the network: input k
+ * is (f_k - in_mean[k]) / in_std[k] with f_k = ln(c) for the 28
+ * concentrations and the count itself for the two open-channel counts;
+ * the prediction is c = exp(y out_std + out_mean).  A normalised input
+ * outside [in_lo - margin, in_hi + margin] trips the range monitor. */
+int cs_mlp_set_normalisation(cs_mlp* m, const float* in_mean, const float* in_std, const float* in_lo,
+                             const float* in_hi, float out_mean, float out_std, float margin);
+/* y[n] for normalised inputs x[n][30] (host arrays). */
+int cs_mlp_forward(cs_mlp* m, const float* x, uint64_t n, float* y);
+/* Gradient of mean((y - t)^2) over n samples, and that mean. */
+int cs_mlp_grad(cs_mlp* m, const float* x, const float* t, uint64_t n, float* grad, double* loss);
+/* Training set (normalised inputs and targets), kept on the device;
+ * minibatches walk a shuffled order drawn from shuffle_seed, reshuffled
+ * every epoch. */
+int cs_mlp_set_data(cs_mlp* m, const float* x, const float* t, uint64_t n, uint64_t shuffle_seed);
+/* `steps` Adam steps (beta 0.9, 0.999, eps 1e-8) of `batch` samples at
+ * learning rate lr; last_loss is the mean squared error of the last batch
+ * before its update. */
+int cs_mlp_train(cs_mlp* m, uint32_t steps, uint32_t batch, float lr, double* last_loss);
 
 /* Scalar flux functions of the plugin table, evaluated on the host with
  * the same code the device runs (ryr_flux boundaryfluxes.c:87-141,

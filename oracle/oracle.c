@@ -58,41 +58,48 @@ struct oracle_sim {
   uint8_t fixed[ND_MAX][NS_MAX];
   int rng;
   uint32_t seed;
+  struct mt_state* mt;       /* this run's Mersenne Twister             */
 };
 
 /* ------------------------------------------------------------------ RNG */
 /* MT19937, Matsumoto & Nishimura (1998), 2002 seeding; mt_ldrand combines
- * two consecutive outputs as in mtwist.h:635-677. */
-static uint32_t g_mt[624];
-static int g_mti = 625;
+ * two consecutive outputs as in mtwist.h:635-677.  The reference keeps one
+ * generator per process (mtwist's default state), which is one per
+ * realisation; here each oracle run owns its generator (ostate), so runs
+ * that step interleaved in one process do not share a stream.  The
+ * oracle_mt_* exports use a process-wide one, for known-answer tests. */
+typedef struct mt_state {
+  uint32_t mt[624];
+  int mti;
+} mt_state;
 
-void oracle_mt_seed(uint32_t seed)
+static void mt_seed(mt_state* g, uint32_t seed)
 {
-  g_mt[0] = seed;
-  for (int i = 1; i < 624; i++) g_mt[i] = 1812433253u * (g_mt[i - 1] ^ (g_mt[i - 1] >> 30)) + (uint32_t)i;
-  g_mti = 624;
+  g->mt[0] = seed;
+  for (int i = 1; i < 624; i++) g->mt[i] = 1812433253u * (g->mt[i - 1] ^ (g->mt[i - 1] >> 30)) + (uint32_t)i;
+  g->mti = 624;
 }
 
-uint32_t oracle_mt_u32(void)
+static uint32_t mt_u32(mt_state* g)
 {
   static const uint32_t mag[2] = {0u, 0x9908b0dfu};
   uint32_t y;
-  if (g_mti >= 624) {
-    if (g_mti == 625) oracle_mt_seed(5489u);
+  if (g->mti >= 624) {
+    if (g->mti == 625) mt_seed(g, 5489u);
     int kk;
     for (kk = 0; kk < 624 - 397; kk++) {
-      y = (g_mt[kk] & 0x80000000u) | (g_mt[kk + 1] & 0x7fffffffu);
-      g_mt[kk] = g_mt[kk + 397] ^ (y >> 1) ^ mag[y & 1u];
+      y = (g->mt[kk] & 0x80000000u) | (g->mt[kk + 1] & 0x7fffffffu);
+      g->mt[kk] = g->mt[kk + 397] ^ (y >> 1) ^ mag[y & 1u];
     }
     for (; kk < 623; kk++) {
-      y = (g_mt[kk] & 0x80000000u) | (g_mt[kk + 1] & 0x7fffffffu);
-      g_mt[kk] = g_mt[kk + (397 - 624)] ^ (y >> 1) ^ mag[y & 1u];
+      y = (g->mt[kk] & 0x80000000u) | (g->mt[kk + 1] & 0x7fffffffu);
+      g->mt[kk] = g->mt[kk + (397 - 624)] ^ (y >> 1) ^ mag[y & 1u];
     }
-    y = (g_mt[623] & 0x80000000u) | (g_mt[0] & 0x7fffffffu);
-    g_mt[623] = g_mt[396] ^ (y >> 1) ^ mag[y & 1u];
-    g_mti = 0;
+    y = (g->mt[623] & 0x80000000u) | (g->mt[0] & 0x7fffffffu);
+    g->mt[623] = g->mt[396] ^ (y >> 1) ^ mag[y & 1u];
+    g->mti = 0;
   }
-  y = g_mt[g_mti++];
+  y = g->mt[g->mti++];
   y ^= (y >> 11);
   y ^= (y << 7) & 0x9d2c5680u;
   y ^= (y << 15) & 0xefc60000u;
@@ -100,12 +107,18 @@ uint32_t oracle_mt_u32(void)
   return y;
 }
 
-double oracle_mt_ldrand(void)
+static double mt_ldrand(mt_state* g)
 {
-  uint64_t a = oracle_mt_u32();
-  uint64_t b = oracle_mt_u32();
+  uint64_t a = mt_u32(g);
+  uint64_t b = mt_u32(g);
   return (double)((a << 32) | b) / 18446744073709551616.0;
 }
+
+static mt_state g_mt = {{0}, 625};
+
+void oracle_mt_seed(uint32_t seed) { mt_seed(&g_mt, seed); }
+uint32_t oracle_mt_u32(void) { return mt_u32(&g_mt); }
+double oracle_mt_ldrand(void) { return mt_ldrand(&g_mt); }
 
 /* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11). */
 void oracle_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
@@ -201,6 +214,7 @@ void oracle_free(oracle_sim* s)
   for (uint32_t b = 0; b < CS_MAX_BOUNDARIES; b++) {
     free(s->bnd[b].from); free(s->bnd[b].to); free(s->bnd[b].open); free(s->bnd[b].ever);
   }
+  free(s->mt);
   free(s);
 }
 
@@ -362,7 +376,9 @@ int oracle_create(const cs_geometry* g, const cs_species_model* m, const cs_opti
   /* seeding and initial channel states (calcium_sparks.c:113-116,
    * boundaryfluxes.c:229-293, :544-605) */
   s->seed = opt->seed_given ? opt->seed : 12345u;
-  oracle_mt_seed(s->seed);
+  s->mt = (mt_state*)calloc(1, sizeof(mt_state));
+  if (!s->mt) { oracle_free(s); return ofail(err, elen, "out of memory"); }
+  mt_seed(s->mt, s->seed);
   if (m->use_ryr && s->ndisc > 0) {
     int rb = -1, pb = -1;
     for (uint32_t b = 0; b < s->nb_geom; b++) {
@@ -388,7 +404,7 @@ int oracle_create(const cs_geometry* g, const cs_species_model* m, const cs_opti
           double c0 = s->u1[0][RB->to[i]];
           double ko = fmax(fmin(pow(c0 / pp[9], pp[2]), pp[3]), pp[4]);
           double kc = fmax(fmin(pow(c0 / pp[0], pp[5]), pp[1]), pp[7]);
-          double r = oracle_mt_ldrand();
+          double r = mt_ldrand(s->mt);
           B->open[i] = (r <= kc / (ko + kc)) ? 0 : 1;
         }
       }
@@ -462,7 +478,7 @@ static void gate_boundary(oracle_sim* s, obnd* B, uint64_t ti)
     double c0 = s->u1[0][B->to[i]];   /* Ca (slot 0) at the to voxel, species.c:2569 */
     double kc = fmax(fmin(pow(c0 / p[0], p[5]), p[1]), p[7]);
     double ko = fmax(fmin(pow(c0 / p[9], p[2]), p[3]), p[4]);
-    double r = s->rng == CS_RNG_MT ? oracle_mt_ldrand()
+    double r = s->rng == CS_RNG_MT ? mt_ldrand(s->mt)
                                    : philox_uniform(s->seed, ti, (uint32_t)B->gate_group, i);
     if (B->open[i] == 0) { if (r <= dts * ko) B->open[i] = 1; }
     else if (B->open[i] == 1) { if (r <= dts * kc) B->open[i] = 0; }

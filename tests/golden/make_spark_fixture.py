@@ -5,8 +5,9 @@ order (mt_seed32new(seed), one mt_ldrand per stochastic channel per gating
 step), one realisation per seed.  The GPU runs the same dyad with its own
 Philox stream and must produce the same distributions (two-sample KS).
 
-Writes tests/golden/spark_stats_fast_dyad.json.  Takes a few minutes on 8
-cores:  python3 tests/golden/make_spark_fixture.py [num_seeds]
+Writes tests/golden/spark_stats_fast_dyad.json (1000 seeds by default,
+the linescan sampled every 0.002 ms).  Takes several minutes on 8 cores:
+python3 tests/golden/make_spark_fixture.py [num_seeds]
 """
 import json
 import os
@@ -25,7 +26,7 @@ import dyad  # noqa: E402
 import sparks  # noqa: E402
 
 # the run every realisation makes (shared with tests/test_gpu_sparks.py)
-RUN = dict(tstop_ms=0.3, save_ms=0.01, species=1, domain="cleft")
+RUN = dict(tstop_ms=0.3, save_ms=0.002, species=1, domain="cleft")
 
 
 def plan(d):
@@ -49,7 +50,7 @@ def one(seed):
 
 
 def main():
-    n = int(sys.argv[1]) if len(sys.argv) > 1 else 300
+    n = int(sys.argv[1]) if len(sys.argv) > 1 else 1000
     with Pool(min(8, os.cpu_count() or 1)) as pool:
         rows = pool.map(one, range(1, n + 1))
     d = dyad.fast_spark_dyad()

@@ -1,0 +1,115 @@
+"""The learned closure on the GPU: schedule replay between a 3 nm and a 12
+nm ensemble of the same 12 nm dyads, the closure's inputs against the
+fields they are read from, training on paired samples, what the trained
+network gains over the plain coarse grid on held-out dyads, and gating on
+its predictions inside an ensemble (with the range monitor)."""
+import sys
+
+import numpy as np
+import pytest
+
+import closure
+import csparks as cs
+import dyad
+import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _dyads(seeds, edge=(192.0, 256.0), ryr=(9, 30)):
+    return [dyad.ensemble_dyad(s, h=12.0, edge_nm=edge, ryr=ryr) for s in seeds]
+
+
+def test_replay_copies_schedules_and_features_read_the_fields():
+    ds = _dyads(range(4))
+    mk = lambda h: cs.Ensemble([(d.geometry, closure.schedule_model(d.model),
+                                 cs.Options(h=h, seed=10 + k, open={"ryr": [d.open_index]}))
+                                for k, d in enumerate(ds)])
+    fine, coarse = mk(3.0), mk(12.0)
+    coarse.set_gating(cs.Ensemble.GATE_REPLAY)
+    ratio = int(round(coarse.dt / fine.dt))
+    assert ratio == 16
+    for _ in range(5):
+        fine.step(ratio)
+        coarse.copy_channel_states(fine)
+        coarse.step(1)
+    for i in range(len(ds)):
+        for f in range(2):
+            np.testing.assert_array_equal(coarse.channel_states(i, f), fine.channel_states(i, f))
+    feats = coarse.closure_features()
+    ca = coarse.channel_ca()
+    o = 0
+    for i, d in enumerate(ds):
+        u = coarse.species(i, 0)
+        n = u.shape
+        st = np.concatenate([coarse.channel_states(i, f) for f in range(2)])
+        chans = np.vstack([d.geometry.boundary("ryr").pairs, d.geometry.boundary("p_ryr").pairs]).astype(int)
+        for c, p in enumerate(chans):
+            fr, to = p[:3], p[3:]
+            want = np.empty(30)
+            for k in range(27):
+                q = [min(max(to[a] + (k // (1, 3, 9)[a]) % 3 - 1, 0), n[a] - 1) for a in range(3)]
+                want[k] = u[q[0], q[1], q[2]]
+            want[27] = u[fr[0], fr[1], fr[2]]
+            cheb = np.max(np.abs(chans[:, 3:] - to), axis=1)
+            want[28] = st[cheb <= 1].sum()
+            want[29] = st[cheb <= 2].sum()
+            np.testing.assert_allclose(feats[o + c], want, rtol=1e-6, atol=1e-30)
+            assert ca[o + c] == pytest.approx(u[to[0], to[1], to[2]], rel=1e-12)
+        o += len(chans)
+    assert o == coarse.num_channels
+    fine.close()
+    coarse.close()
+
+
+@pytest.fixture(scope="module")
+def trained():
+    train = closure.paired_samples(_dyads(range(12)), list(range(12)), coarse_steps=300, sample_every=2)
+    _say(f"training set: {len(train.ca_fine)} samples from {train.fine_steps} fine / {train.coarse_steps} coarse steps")
+    net, norm = closure.train(train, mo.init_params(0), steps=2500, batch=1024, lr=2e-3)
+    yield net, norm
+    net.close()
+
+
+def test_trained_closure_beats_the_plain_coarse_grid(trained):
+    """On held-out dyads, the closure's log-Ca error at the channel mouth
+    is a fraction of the error of gating on the coarse cleft voxel."""
+    net, norm = trained
+    held = closure.paired_samples(_dyads(range(100, 106)), list(range(100, 106)), coarse_steps=200, sample_every=4)
+    e_closure, e_plain = closure.log_errors(net, norm, held)
+    opened = held.raw[:, 28] > 0
+    _say(f"held-out log-Ca MSE: closure {e_closure:.4f}, plain coarse {e_plain:.4f} "
+         f"({len(held.ca_fine)} samples, {opened.mean():.0%} with an open channel nearby)")
+    assert e_closure < 0.2 * e_plain
+
+
+def test_closure_gating_uses_the_network_and_monitors_its_range(trained):
+    net, norm = trained
+    ds = _dyads(range(200, 204))
+    ens = cs.Ensemble([(d.geometry, d.model, cs.Options(h=12.0, seed=20 + k, open={"ryr": [d.open_index]}))
+                       for k, d in enumerate(ds)], instances=list(range(4)))
+    ens.set_gating(cs.Ensemble.GATE_CLOSURE, net)
+    ens.step(50)
+    # the predictions of a step come from the state and channels at its start
+    feats = ens.closure_features()
+    ens.step(1)
+    pred = ens.closure_predictions()
+    want = norm.concentration(net.forward(norm.inputs(feats)))
+    assert np.all(np.isfinite(pred)) and np.all(pred > 0)
+    np.testing.assert_allclose(pred, want, rtol=0.03)
+    assert ens.monitor().sum() == 0, "inputs of these dyads lie inside the training range"
+    # a range no input can meet: every evaluation is flagged
+    tight = closure.Normalisation(norm.in_mean, norm.in_std, np.full(30, 1e9, np.float32),
+                                  np.full(30, 1e9, np.float32), norm.out_mean, norm.out_std, 0.0)
+    tight.apply(net)
+    ens.step(3)
+    flags = ens.monitor()
+    per = np.bincount(ens.channel_owner(), minlength=4)
+    np.testing.assert_array_equal(flags, 3 * per)
+    norm.apply(net)
+    ens.close()

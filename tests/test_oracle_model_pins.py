@@ -251,3 +251,37 @@ def test_coupled_step_conserves_total_calcium():
     assert t0[jsr].sum() - t1[jsr].sum() > 1e-6 * t0[jsr].sum()
     assert t1[nsr].sum() > t0[nsr].sum()
     o.close()
+
+
+def test_interleaved_oracle_runs_keep_their_own_generators():
+    """Each oracle run draws from its own Mersenne Twister: two runs stepped
+    alternately give the channel trajectories each gives alone."""
+    import dataclasses
+    d = dyad.small_dyad(seed=4)
+    kin = dict(Kd_open=2.0, k_max_open=300.0, k_min_close=300.0, k_max_close=300.0)
+    m = dataclasses.replace(d.model, ryr=dataclasses.replace(d.model.ryr, **kin),
+                            p_ryr=dataclasses.replace(d.model.p_ryr, **kin))
+    o1 = cs.Options(seed=11, rng="mt", open={"ryr": [d.open_index]})
+    o2 = cs.Options(seed=12, rng="mt", open={"ryr": [d.open_index]})
+
+    def alone(o):
+        r = op.Oracle(d.geometry, m, o)
+        out = []
+        for _ in range(30):
+            r.step(20)
+            out.append(np.concatenate([r.states(0), r.states(1)]))
+        r.close()
+        return out
+
+    a1, a2 = alone(o1), alone(o2)
+    r1, r2 = op.Oracle(d.geometry, m, o1), op.Oracle(d.geometry, m, o2)
+    flips = 0
+    for k in range(30):
+        r1.step(20)
+        r2.step(20)
+        np.testing.assert_array_equal(np.concatenate([r1.states(0), r1.states(1)]), a1[k])
+        np.testing.assert_array_equal(np.concatenate([r2.states(0), r2.states(1)]), a2[k])
+        flips += int(np.sum(a1[k] != a1[k - 1])) if k else 0
+    assert flips > 0
+    r1.close()
+    r2.close()
