@@ -40,6 +40,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -1237,6 +1238,40 @@ k_linescan_cols(const T* __restrict__ u, uint32_t s, const uint8_t* __restrict__
   }
 }
 
+/* Projection on the scan axis from the partial sums of k_linescan_cols
+ * (axis 0, gyc row chunks per plane) or k_linescan_rows (axes 1, 2), one
+ * thread per output, in cs_linescan's order: axis 0 proj[x] = sum_z
+ * wz[z] (sum_c part[z][c][x]); axis 1 proj[y] = sum_z wz[z] rows[z][y];
+ * axis 2 proj[z] = sum_y wy[y] rows[z][y].  w holds wx | wy | wz. */
+__global__ void __launch_bounds__(256)
+k_linescan_proj(const double* __restrict__ part, const double* __restrict__ w, Geo g, int gyc, uint32_t axis,
+                double* __restrict__ proj)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const double* wy = w + g.nx;
+  const double* wz = wy + g.ny;
+  if (axis == 0) {
+    if (k >= g.nx) return;
+    double acc = 0.0;
+    for (int z = 0; z < g.nz; z++) {
+      double col = 0.0;
+      for (int c = 0; c < gyc; c++) col += part[((uint64_t)z * gyc + c) * g.nx + k];
+      acc += wz[z] * col;
+    }
+    proj[k] = acc;
+  } else if (axis == 1) {
+    if (k >= g.ny) return;
+    double acc = 0.0;
+    for (int z = 0; z < g.nz; z++) acc += wz[z] * part[(uint64_t)z * g.ny + k];
+    proj[k] = acc;
+  } else {
+    if (k >= g.nz) return;
+    double acc = 0.0;
+    for (int y = 0; y < g.ny; y++) acc += wy[y] * part[(uint64_t)k * g.ny + y];
+    proj[k] = acc;
+  }
+}
+
 /* Plain copy of a known number of bytes (16 B per lane per iteration),
  * used to calibrate memory-traffic counters against algorithmic bytes. */
 __global__ void k_calibration_copy(const uint4* __restrict__ a, uint4* __restrict__ b, uint64_t n16)
@@ -1255,6 +1290,42 @@ __global__ void k_fill_init(T* __restrict__ u, Geo g)
   const uint64_t n = g.nv * g.ns;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     u[i] = T(0);
+}
+
+}  // namespace
+
+/* Scratch of cs_linescan_noalloc: the axis weights, the partial sums and
+ * the projection of one call, so that call allocates nothing.  Each GPU
+ * the library runs on holds its own copy until the process ends.  Host
+ * code never passes this name to a kernel: it takes the device address
+ * from hipGetSymbolAddress once per device (scratch_on_device). */
+static __device__ __align__(256) double scratch[CS_LINESCAN_NOALLOC_MAX_DOUBLES];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[MAX_DEVICES] = {nullptr};
+
+/* Device address of `scratch` on the current device, looked up once per
+ * device (nullptr with the error recorded on failure). */
+double* scratch_on_device(int device)
+{
+  if (device < 0 || device >= MAX_DEVICES) {
+    set_err(CS_EDEVICE, "device ordinal %d beyond %d", device, MAX_DEVICES);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      set_err(CS_EDEVICE, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_ptr[device] = (double*)ptr;
+  }
+  return g_scratch_ptr[device];
 }
 
 }  // namespace
@@ -1293,6 +1364,7 @@ struct cs_context {
   double* d_stats = nullptr;
   double* d_lweights = nullptr;   /* linescan weights, nx + ny + nz        */
   double* d_lpart = nullptr;      /* linescan partial sums                  */
+  double* scratch = nullptr;      /* this device's `scratch` (no-alloc scan)  */
   /* host copies */
   cs_problem prob{};
   std::vector<uint32_t> state_off, npairs;
@@ -2007,6 +2079,8 @@ int cs_create(const cs_problem* p, cs_context** out)
   if (hipGetDevice(&c->device) != hipSuccess) return bail(set_err(CS_EDEVICE, "no HIP device"));
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(set_err(CS_EDEVICE, "hipStreamCreate failed"));
+  c->scratch = scratch_on_device(c->device);
+  if (!c->scratch) return bail(CS_EDEVICE);
 
   /* domain ids in device order with the uniform-neighbourhood bit */
   std::vector<uint8_t> ids(g.nv);
@@ -2291,6 +2365,65 @@ int cs_linescan(cs_context* c, uint32_t species, uint32_t axis, const double* wx
       proj[z] = acc;
     }
   }
+  return CS_OK;
+}
+
+int cs_linescan_noalloc(cs_context* c, uint32_t species, uint32_t axis, const double* wx, const double* wy,
+                        const double* wz, uint32_t domain_mask, double* proj)
+{
+  if (!c || !proj) return set_err(CS_EINVAL, "NULL argument");
+  if (species >= c->ns) return set_err(CS_ERANGE, "species %u out of range", species);
+  if (axis > 2) return set_err(CS_EINVAL, "axis must be 0, 1 or 2");
+  if ((axis != 0 && !wx) || (axis != 1 && !wy) || (axis != 2 && !wz))
+    return set_err(CS_EINVAL, "the weights of both transverse axes are required");
+  if (!c->scratch) return set_err(CS_EDEVICE, "no scratch on this device");
+  const Geo& g = c->geo;
+  const int ychunk = 64;
+  const int gyc = (g.ny + ychunk - 1) / ychunk;
+  /* scratch segments, each starting on 256 bytes: weights | partial sums |
+   * projection */
+  auto up32 = [](uint64_t n) { return (n + 31) / 32 * 32; };
+  const uint64_t nw = (uint64_t)g.nx + g.ny + g.nz;
+  const uint64_t np = axis == 0 ? (uint64_t)g.nz * gyc * g.nx : (uint64_t)g.ny * g.nz;
+  const uint64_t nk = axis == 0 ? g.nx : axis == 1 ? g.ny : g.nz;
+  const uint64_t need = up32(nw) + up32(np) + up32(nk);
+  if (need > CS_LINESCAN_NOALLOC_MAX_DOUBLES)
+    return set_err(CS_ERANGE, "linescan needs %llu scratch doubles, beyond the %u this entry point holds",
+                   (unsigned long long)need, (unsigned)CS_LINESCAN_NOALLOC_MAX_DOUBLES);
+  HIP_TRY(hipSetDevice(c->device));
+  double* dw = c->scratch;
+  double* dpart = dw + up32(nw);
+  double* dproj = dpart + up32(np);
+  /* weights the kernels read; the scan axis's own weights are not read */
+  if (wx && axis != 0) HIP_TRY(hipMemcpyAsync(dw, wx, sizeof(double) * g.nx, hipMemcpyHostToDevice, c->stream));
+  if (wy && axis != 1)
+    HIP_TRY(hipMemcpyAsync(dw + g.nx, wy, sizeof(double) * g.ny, hipMemcpyHostToDevice, c->stream));
+  if (wz && axis != 2)
+    HIP_TRY(hipMemcpyAsync(dw + g.nx + g.ny, wz, sizeof(double) * g.nz, hipMemcpyHostToDevice, c->stream));
+  const void* us = c->d_u[c->cur];
+  if (axis == 0) {
+    if (c->esize == 8)
+      hipLaunchKernelGGL(k_linescan_cols<double>, dim3(g.nz, gyc), dim3(256), 0, c->stream, (const double*)us,
+                         species, c->d_ids, (const DevParams<double>*)c->d_params, g, domain_mask, dw + g.nx,
+                         ychunk, dpart);
+    else
+      hipLaunchKernelGGL(k_linescan_cols<float>, dim3(g.nz, gyc), dim3(256), 0, c->stream, (const float*)us,
+                         species, c->d_ids, (const DevParams<float>*)c->d_params, g, domain_mask, dw + g.nx,
+                         ychunk, dpart);
+  } else {
+    const unsigned blocks = (unsigned)(((uint64_t)g.ny * g.nz + 3) / 4);
+    if (c->esize == 8)
+      hipLaunchKernelGGL(k_linescan_rows<double>, dim3(blocks), dim3(256), 0, c->stream, (const double*)us, species,
+                         c->d_ids, (const DevParams<double>*)c->d_params, g, domain_mask, dw, dpart);
+    else
+      hipLaunchKernelGGL(k_linescan_rows<float>, dim3(blocks), dim3(256), 0, c->stream, (const float*)us, species,
+                         c->d_ids, (const DevParams<float>*)c->d_params, g, domain_mask, dw, dpart);
+  }
+  hipLaunchKernelGGL(k_linescan_proj, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, c->stream, dpart, dw, g,
+                     gyc, axis, dproj);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(proj, dproj, sizeof(double) * nk, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return CS_OK;
 }
 

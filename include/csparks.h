@@ -159,6 +159,21 @@ int cs_reduce_stats(cs_context* ctx, double* min_values, double* max_values,
 int cs_linescan(cs_context* ctx, uint32_t species, uint32_t axis, const double* wx,
                 const double* wy, const double* wz, uint32_t domain_mask, double* proj);
 
+/* cs_linescan that allocates nothing: no hipMalloc, hipFree or other
+ * allocating call, for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid it).  No
+ * counterpart in the reference.  The weights, the partial sums and the
+ * projection live in a fixed device array of CS_LINESCAN_NOALLOC_MAX_DOUBLES
+ * doubles (8 MiB) per GPU, held until the process ends; a call needs
+ * nx+ny+nz, plus nz*ceil(ny/64)*nx (axis 0) or ny*nz (axes 1, 2), plus
+ * n[axis] doubles, each rounded up to 32, and fails with CS_ERANGE beyond
+ * that.  Every call on one GPU shares the array: calls that overlap on
+ * one GPU (other host threads, other contexts' streams) race.  Results
+ * equal cs_linescan's up to rounding of the final sums. */
+#define CS_LINESCAN_NOALLOC_MAX_DOUBLES (1u << 20)
+int cs_linescan_noalloc(cs_context* ctx, uint32_t species, uint32_t axis, const double* wx,
+                        const double* wy, const double* wz, uint32_t domain_mask, double* proj);
+
 /* Channel states of discrete flux f (0 closed, 1 open). */
 int cs_get_channel_states(cs_context* ctx, uint32_t flux, uint8_t* states);
 int cs_set_channel_states(cs_context* ctx, uint32_t flux, const uint8_t* states);
