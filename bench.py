@@ -58,7 +58,7 @@ WORKLOAD_KEY = "cluster_dyad"
 CLOSURE_PARAMS = os.path.join(REPO, "hpc-ca-phospattern_amd", "closure_params.npz")
 ENSEMBLE_LAUNCH = {"rows": 8, "zchunk": 8, "prefetch": 1, "nontemporal": 0}
 LAUNCH_KEYS = ["rows", "zchunk", "xcd_remap", "prefetch", "nontemporal", "lds_tile", "linear", "yblock",
-               "fuse", "fuse_zchunk", "fuse_xcd"]
+               "fuse", "fuse_zchunk", "fuse_xcd", "fuse_fixed"]
 
 
 def log(msg):

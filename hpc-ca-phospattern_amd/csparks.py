@@ -174,7 +174,7 @@ class CLaunchConfig(C.Structure):
     _fields_ = [("rows", C.c_uint32), ("zchunk", C.c_uint32), ("xcd_remap", C.c_uint32),
                 ("prefetch", C.c_uint32), ("nontemporal", C.c_uint32), ("lds_tile", C.c_uint32),
                 ("linear", C.c_uint32), ("yblock", C.c_uint32), ("fuse", C.c_uint32),
-                ("fuse_zchunk", C.c_uint32), ("fuse_xcd", C.c_uint32)]
+                ("fuse_zchunk", C.c_uint32), ("fuse_xcd", C.c_uint32), ("fuse_fixed", C.c_uint32)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

@@ -82,6 +82,7 @@ struct LaunchCfg {
   int fuse;      /* 1: pairs of eligible steps run as one k_step2 pass     */
   int fz;        /* planes per block of k_step2                             */
   int fx;        /* xcd_tile map of k_step2's blocks                        */
+  int ff;        /* 1: k_step2 compiled for the pass's step masks, if any   */
 };
 
 template <typename T, int NS, int PF, bool NT>
@@ -645,7 +646,14 @@ struct F2Step {
  * voxel is kept in its slot: step 2's gating and fluxes need only those
  * values, and its fluxes add to the result after the pass (k_bridge),
  * since no other voxel's step-2 update reads them. */
-template <typename T, int NS, int WY, bool NT>
+/* Step masks of a fused pass fixed at compile time: FM1/FM2 are the
+ * diffusing species of its two steps and FR the reaction mode of both, or
+ * F2_DYN for values taken from s1/s2 at run time.  With fixed masks the
+ * species loops keep only the species that diffuse, where run-time bits
+ * leave every species' code in the loop behind a test. */
+constexpr uint32_t F2_DYN = 0xFFFFFFFFu;
+
+template <typename T, int NS, int WY, bool NT, uint32_t FM1 = F2_DYN, uint32_t FM2 = F2_DYN, uint32_t FR = F2_DYN>
 __global__ void __launch_bounds__(BX * WY)
 k_step2(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__ ids,
         const DevParams<T>* __restrict__ P, Geo g, F2Voxels fv, const double* __restrict__ fsum,
@@ -700,7 +708,8 @@ k_step2(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__
   auto iofs = [&](int zz) { return (uint32_t)(min(max(zz, zb), ze - 1) - zb) * (uint32_t)plane; };
 
   const uint32_t react_nb = (uint32_t)P->nb;
-  uint32_t dm1 = s1.dmask, dm2 = s2.dmask, re1 = s1.react, re2 = s2.react, rbound = P->rbound;
+  uint32_t dm1 = FM1 != F2_DYN ? FM1 : s1.dmask, dm2 = FM2 != F2_DYN ? FM2 : s2.dmask;
+  uint32_t re1 = FR != F2_DYN ? FR : s1.react, re2 = FR != F2_DYN ? FR : s2.react, rbound = P->rbound;
   __shared__ uint32_t msk[5];
   if (lane == 0 && wy == 0) {
     msk[0] = dm1;
@@ -866,10 +875,12 @@ k_step2(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__
      * orders the first read after the write): each species test is then a
      * bit test of one scalar register, not a flag hoisted out of the loop,
      * of which there are too many to keep in scalar registers */
-    dm1 = __builtin_amdgcn_readfirstlane(msk[0]);
-    dm2 = __builtin_amdgcn_readfirstlane(msk[1]);
-    re1 = __builtin_amdgcn_readfirstlane(msk[2]);
-    re2 = __builtin_amdgcn_readfirstlane(msk[3]);
+    if constexpr (FM1 == F2_DYN) dm1 = __builtin_amdgcn_readfirstlane(msk[0]);
+    if constexpr (FM2 == F2_DYN) dm2 = __builtin_amdgcn_readfirstlane(msk[1]);
+    if constexpr (FR == F2_DYN) {
+      re1 = __builtin_amdgcn_readfirstlane(msk[2]);
+      re2 = __builtin_amdgcn_readfirstlane(msk[3]);
+    }
     rbound = __builtin_amdgcn_readfirstlane(msk[4]);
     const bool below = z < nz;
     if (below) {
@@ -1427,7 +1438,7 @@ int max_fused_zchunk(const Geo& g, size_t esize)
  * chunk and XCD groups below. */
 LaunchCfg default_launch(const Geo& g, size_t esize, int cus)
 {
-  LaunchCfg L{4, 8, 16, 1, 1, 0, 1, 0, 0, 40, 8};
+  LaunchCfg L{4, 8, 16, 1, 1, 0, 1, 0, 0, 40, 8, 1};
   L.zchunk = std::min(L.zchunk, max_zchunk(g, esize));
   /* k_step2: split z into k equal chunks; with one block per CU at a
    * time, the pass takes ceil(blocks / CUs) rounds of (chunk + 2) planes
@@ -1716,9 +1727,9 @@ bool can_fuse(const cs_context* c, const LaunchCfg& L, uint64_t ti)
   return (c->fsm & ~diffusing(c, ti)) == 0 && (c->fsm & ~diffusing(c, ti + 1)) == 0;
 }
 
-template <typename T, int NS>
-void launch_step2_ns(cs_context* c, const LaunchCfg& L, const void* uin, void* uout, const F2Step& a,
-                     const F2Step& b)
+template <typename T, int NS, uint32_t FM1 = F2_DYN, uint32_t FM2 = F2_DYN, uint32_t FR = F2_DYN>
+void launch_step2_k(cs_context* c, const LaunchCfg& L, const void* uin, void* uout, const F2Step& a,
+                    const F2Step& b)
 {
   constexpr int WY = sizeof(T) == 4 ? 16 : 8;
   const Geo& g = c->geo;
@@ -1727,10 +1738,40 @@ void launch_step2_ns(cs_context* c, const LaunchCfg& L, const void* uin, void* u
   const int tz = (g.nz + zc - 1) / zc;
   const int nblocks = tx * ty * tz;
   const F2Voxels fv{c->d_f2_col, c->d_f2_z, c->d_f2_row};
-  hipLaunchKernelGGL((k_step2<T, NS, WY, true>), dim3(xcd_launch(nblocks, L.fx)), dim3(BX, WY), 0, c->stream,
-                     (const T*)uin, (T*)uout, c->d_ids, (const DevParams<T>*)c->d_params, g, fv,
+  hipLaunchKernelGGL((k_step2<T, NS, WY, true, FM1, FM2, FR>), dim3(xcd_launch(nblocks, L.fx)), dim3(BX, WY), 0,
+                     c->stream, (const T*)uin, (T*)uout, c->d_ids, (const DevParams<T>*)c->d_params, g, fv,
                      (const double*)c->d_f2_fsum, (const uint8_t*)c->d_f2_fany, (T*)c->d_f2_l1, a, b, zc, tx, ty,
                      nblocks, L.fx);
+}
+
+/* Pairs of step masks compiled with fixed masks, for the species layout of
+ * the reference's species files (parameters/*.h5: Ca, Fluo, CMDN-CSQN,
+ * ATP, TRPN): Ca and ATP diffuse on every step (0x9), the dye and CMDN on
+ * their own multiples of dt (0x2, 0x4), TRPN never.  These are every pair
+ * of consecutive steps when the two slow species' substeps are at least 2;
+ * the reaction runs species-major on both steps.  Other pairs, species
+ * counts and FP64 run the kernel with run-time masks. */
+template <typename T, int NS>
+void launch_step2_ns(cs_context* c, const LaunchCfg& L, const void* uin, void* uout, const F2Step& a,
+                     const F2Step& b)
+{
+  if constexpr (sizeof(T) == 4 && NS == 5) {
+    if (L.ff && a.react == 2 && b.react == 2) {
+      switch ((a.dmask << 8) | b.dmask) {
+        case 0x0909: return launch_step2_k<T, NS, 0x9, 0x9, 2>(c, L, uin, uout, a, b);
+        case 0x0B09: return launch_step2_k<T, NS, 0xB, 0x9, 2>(c, L, uin, uout, a, b);
+        case 0x090B: return launch_step2_k<T, NS, 0x9, 0xB, 2>(c, L, uin, uout, a, b);
+        case 0x0D09: return launch_step2_k<T, NS, 0xD, 0x9, 2>(c, L, uin, uout, a, b);
+        case 0x090D: return launch_step2_k<T, NS, 0x9, 0xD, 2>(c, L, uin, uout, a, b);
+        case 0x0F09: return launch_step2_k<T, NS, 0xF, 0x9, 2>(c, L, uin, uout, a, b);
+        case 0x090F: return launch_step2_k<T, NS, 0x9, 0xF, 2>(c, L, uin, uout, a, b);
+        case 0x0B0D: return launch_step2_k<T, NS, 0xB, 0xD, 2>(c, L, uin, uout, a, b);
+        case 0x0D0B: return launch_step2_k<T, NS, 0xD, 0xB, 2>(c, L, uin, uout, a, b);
+        default: break;
+      }
+    }
+  }
+  launch_step2_k<T, NS>(c, L, uin, uout, a, b);
 }
 
 template <typename T>
@@ -2483,6 +2524,7 @@ int cs_get_launch_config(const cs_context* c, cs_launch_config* out)
   out->fuse = (uint32_t)c->launch.fuse;
   out->fuse_zchunk = (uint32_t)c->launch.fz;
   out->fuse_xcd = (uint32_t)c->launch.fx;
+  out->fuse_fixed = (uint32_t)c->launch.ff;
   return CS_OK;
 }
 
@@ -2522,6 +2564,7 @@ int cs_set_launch_config(cs_context* c, const cs_launch_config* in)
   if (in->fuse > 1) return set_err(CS_EINVAL, "fuse must be 0 or 1");
   if (in->fuse && !c->f2_ok) return set_err(CS_EINVAL, "fused passes unavailable: %s", c->f2_why.c_str());
   if (in->fuse_xcd > 64) return set_err(CS_EINVAL, "fuse_xcd is 0, 1 or a group size up to 64");
+  if (in->fuse_fixed > 1) return set_err(CS_EINVAL, "fuse_fixed must be 0 or 1");
   const int fzmax = max_fused_zchunk(c->geo, c->esize);
   if (in->fuse_zchunk < 1) return set_err(CS_EINVAL, "fuse_zchunk must be >= 1");
   if (fzmax >= 1 && (int)std::min<uint32_t>(in->fuse_zchunk, (uint32_t)c->geo.nz) > fzmax)
@@ -2529,6 +2572,7 @@ int cs_set_launch_config(cs_context* c, const cs_launch_config* in)
   L.fuse = (int)in->fuse;
   L.fz = (int)std::min<uint32_t>(in->fuse_zchunk, (uint32_t)c->geo.nz);
   L.fx = (int)in->fuse_xcd;
+  L.ff = (int)in->fuse_fixed;
   c->launch = L;
   return CS_OK;
 }
@@ -2621,6 +2665,7 @@ int cs_autotune(cs_context* c, uint64_t first_time_ind, uint32_t reps, cs_launch
   winner.fuse = 0;
   winner.fz = c->launch.fz;
   winner.fx = c->launch.fx;
+  winner.ff = c->launch.ff;
   /* fused two-step passes: the pairs (ti, ti+1) of two periods that may
    * fuse, grouped by their patterns, without gating (so the channel
    * states and recruitment flags stay as they are); a pass counts as two

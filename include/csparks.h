@@ -225,11 +225,15 @@ int cs_fused_available(const cs_context* ctx, char* why, size_t why_len);
  * tile, so the intermediate state stays on chip and the fields cross HBM
  * once per two steps; fuse_zchunk is its planes per block and fuse_xcd
  * its tile order (as xcd_remap).  The other steps, and every step with
- * fuse = 0, run one at a time.  No reference counterpart: the reference's
+ * fuse = 0, run one at a time.  fuse_fixed = 1 (the default) runs the
+ * pairs of step masks of the reference's five-slot species layout (Ca,
+ * dye, CMDN-CSQN, ATP, TRPN) in FP32 with kernels compiled for those
+ * masks; 0 runs every pass with the masks read at run time (same
+ * results to FP32 rounding).  No reference counterpart: the reference's
  * OpenMP/AVX-512 loop has no launch shape (kernel_generic.cpp:561-600). */
 typedef struct cs_launch_config {
   uint32_t rows, zchunk, xcd_remap, prefetch, nontemporal, lds_tile, linear, yblock;
-  uint32_t fuse, fuse_zchunk, fuse_xcd;
+  uint32_t fuse, fuse_zchunk, fuse_xcd, fuse_fixed;
 } cs_launch_config;
 
 int cs_get_launch_config(const cs_context* ctx, cs_launch_config* cfg);

@@ -47,6 +47,31 @@ def species_close(a: np.ndarray, b: np.ndarray, rtol: float):
     return not bad.any(), float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-6 * max(scale, 1e-300))))
 
 
+DEV_F32 = 1e-4   # FP32 fields against FP64: error per species over its largest deviation from rest
+
+
+def rest_field(d: dyad.Dyad, s: int) -> np.ndarray:
+    """Resting value of species s per voxel of the dyad's own grid (the
+    species file's init value of the voxel's domain)."""
+    return np.asarray(d.model.init)[d.geometry.voxels, s]
+
+
+def deviation_error(got: np.ndarray, ref: np.ndarray, rest: np.ndarray):
+    """(max |got - ref|, max |ref - rest|), rest per voxel."""
+    dev = float(np.max(np.abs(ref - rest)))
+    err = float(np.max(np.abs(got - ref)))
+    return err, dev
+
+
+def deviation_close(got: np.ndarray, ref: np.ndarray, rest: np.ndarray, rtol: float):
+    """FP32 fields hold deviations from rest, so their error scales with the
+    deviation, not with the value: |got - ref| <= rtol x max |ref - rest|,
+    plus 1e-12 of the resting value for a species that has not moved."""
+    err, dev = deviation_error(got, ref, rest)
+    floor = 1e-12 * float(np.max(np.abs(rest)))
+    return err <= rtol * dev + floor, err / max(dev, 1e-300), err, dev
+
+
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 

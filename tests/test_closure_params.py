@@ -1,0 +1,45 @@
+"""The shipped closure parameters (hpc-ca-phospattern_amd/closure_params.npz,
+written by train_closure.py): a plain .npz that loads without pickles,
+holds a network of the C ABI's size with a usable normalisation, and
+records a held-out error far below the plain coarse grid's."""
+import os
+
+import numpy as np
+
+import mlp_oracle as mo
+
+PARAMS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                      "hpc-ca-phospattern_amd", "closure_params.npz")
+
+
+def _load():
+    return np.load(PARAMS, allow_pickle=False)
+
+
+def test_shapes_and_values():
+    z = _load()
+    assert z["params"].shape == (mo.N_PARAMS,) == (6337,)
+    for k in ("in_mean", "in_std", "in_lo", "in_hi"):
+        assert z[k].shape == (30,)
+        assert np.all(np.isfinite(z[k]))
+    assert np.all(np.isfinite(z["params"]))
+    assert np.all(z["in_std"] > 0)
+    assert np.all(z["in_lo"] <= 0) and np.all(z["in_hi"] >= 0), "the training mean lies inside the range"
+    out_mean, out_std, margin = z["out"]
+    assert out_std > 0 and margin >= 0
+    assert int(z["samples"][0]) > 100000
+
+
+def test_recorded_held_out_error():
+    e_closure, e_plain = _load()["held_out_mse"]
+    assert e_closure < 0.05 * e_plain
+
+
+def test_oracle_forward_is_finite_over_the_training_range():
+    z = _load()
+    rng = np.random.default_rng(0)
+    x = rng.uniform(z["in_lo"], z["in_hi"], (256, 30))
+    y = mo.forward(z["params"], x)[0]
+    assert np.all(np.isfinite(y))
+    # standardised log concentrations: a few standard deviations at most
+    assert np.max(np.abs(y)) < 20.0

@@ -113,3 +113,20 @@ def test_closure_gating_uses_the_network_and_monitors_its_range(trained):
     np.testing.assert_array_equal(flags, 3 * per)
     norm.apply(net)
     ens.close()
+
+
+def test_shipped_closure_on_unseen_dyads():
+    """The committed parameters (train_closure.py: 48 dyads, seeds 0-47)
+    on dyads of other seeds and the same size range."""
+    import os
+    path = os.path.join(os.path.dirname(closure.__file__), "closure_params.npz")
+    net, norm, trained = closure.load(path)
+    assert trained
+    seeds = list(range(30000, 30006))
+    held = closure.paired_samples(_dyads(seeds, edge=(256.0, 1024.0), ryr=(9, 150)), seeds,
+                                  coarse_steps=200, sample_every=4)
+    e_closure, e_plain = closure.log_errors(net, norm, held)
+    _say(f"shipped closure, unseen dyads: log-Ca MSE closure {e_closure:.5f}, plain coarse {e_plain:.5f} "
+         f"({len(held.ca_fine)} samples)")
+    assert e_closure < 0.05 * e_plain
+    net.close()

@@ -15,29 +15,17 @@ import pytest
 
 import csparks as cs
 import dyad
-from helpers import species_close
+from helpers import DEV_F32, deviation_error, rest_field, species_close
 from oracle_py import Oracle
 
 pytestmark = pytest.mark.gpu
 
 RTOL_F64 = 1e-10
-DEV_F32 = 1e-4
 
 
 def _say(msg):
     print(msg, flush=True)
     sys.stdout.flush()
-
-
-def deviation_error(got: np.ndarray, ref: np.ndarray, rest: np.ndarray):
-    """max |got - ref| over the largest |ref - rest| (rest per voxel)."""
-    dev = float(np.max(np.abs(ref - rest)))
-    err = float(np.max(np.abs(got - ref)))
-    return err, dev
-
-
-def rest_field(d, s):
-    return np.asarray(d.model.init)[d.geometry.voxels, s]
 
 
 @pytest.fixture(scope="module")

@@ -13,13 +13,12 @@ import pytest
 
 import csparks as cs
 import dyad
-from helpers import species_close, with_serca
+from helpers import DEV_F32, deviation_close, rest_field, species_close, with_serca
 from oracle_py import Oracle
 
 pytestmark = pytest.mark.gpu
 
 RTOL_F64 = 1e-10
-RTOL_F32 = 2e-4
 
 
 def _say(msg):
@@ -180,10 +179,10 @@ def test_fp32_against_fp64_oracle():
     orc.step(300)
     errs = []
     for s in range(orc.ns):
-        ok, err = species_close(sim.species(s), orc.species(s), RTOL_F32)
-        errs.append(err)
-        assert ok, f"species {s}: fp32 max rel err {err:.3e} > {RTOL_F32}"
-    _say("fp32 max rel err per species: " + ", ".join(f"{e:.2e}" for e in errs))
+        ok, rel, err, dev = deviation_close(sim.species(s), orc.species(s), rest_field(d, s), DEV_F32)
+        errs.append(rel)
+        assert ok, f"species {s}: fp32 error {err:.3e} > {DEV_F32} x max deviation {dev:.3e}"
+    _say("fp32 error / max deviation from rest, per species: " + ", ".join(f"{e:.2e}" for e in errs))
     sim.close()
 
 
@@ -255,16 +254,23 @@ def test_linescan_matches_oracle(precision):
     without the domain rule."""
     d = with_serca(dyad.small_dyad(seed=16))
     opts = cs.Options(seed=2, open={"ryr": [d.open_index]}, t_close={"ryr": 1.0}, precision=precision)
+    rest = Oracle(d.geometry, d.model, opts)     # t = 0: every field at rest
     sim, orc = _run_pair(d, opts, 60)
-    tol = RTOL_F64 if precision == "f64" else RTOL_F32
     for s, axis, offsets, nd in [(1, 0, (0, 2, -3), 0), (0, 1, (1, 0, 2), 2), (1, 2, (-2, 3, 0), 1),
                                  (2, 2, (0, 0, 0), 0), (0, 0, (0, -4, 1), 3)]:
         _say(f"linescan species {s} axis {axis} offsets {offsets} domains {nd}")
         got = sim.linescan(s, axis, offsets, d.model.conv, nd)
         want = orc.linescan(s, axis, offsets, d.model.conv, nd)
-        np.testing.assert_allclose(got, want, rtol=tol, atol=tol * float(np.max(np.abs(want))))
+        if precision == "f64":
+            np.testing.assert_allclose(got, want, rtol=RTOL_F64, atol=RTOL_F64 * float(np.max(np.abs(want))))
+        else:
+            # FP32 fields resolve the deviation from rest: so does their linescan
+            ok, rel, err, dev = deviation_close(got, want, rest.linescan(s, axis, offsets, d.model.conv, nd), DEV_F32)
+            _say(f"  fp32 error {err:.3e} = {rel:.2e} of the largest change {dev:.3e}")
+            assert ok
     sim.close()
     orc.close()
+    rest.close()
 
 
 def test_linescan_wide_grid():
