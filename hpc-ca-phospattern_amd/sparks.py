@@ -120,9 +120,10 @@ def realise_batch(runs, nsteps: int, save_every: int, species: int, offsets: Seq
 
 
 def realise_ensemble(ens, nsteps: int, save_every: int, species: int, offsets: Sequence[int],
-                     conv: Sequence[float], axis: int = 0, num_domains: int = 0):
+                     conv: Sequence[float], axis: int = 0, num_domains: int = 0, progress=None):
     """realise() for every member of a csparks.Ensemble: the members step
-    together and one batched linescan per output serves them all."""
+    together and one batched linescan per output serves them all.
+    progress(done, nsteps), if given, is called after every output."""
     f0 = ens.linescans(species, axis, offsets, conv, num_domains)
     times, peaks = [0.0], [[0.0] for _ in range(ens.count)]
     done = 0
@@ -133,6 +134,8 @@ def realise_ensemble(ens, nsteps: int, save_every: int, species: int, offsets: S
         times.append(done * ens.dt)
         for i, f in enumerate(ens.linescans(species, axis, offsets, conv, num_domains)):
             peaks[i].append(float(np.max((f - f0[i]) / f0[i])))
+        if progress is not None:
+            progress(done, nsteps)
     _, _, recruited = ens.summary()
     return [SparkStats(peak=float(np.max(p)), duration=fdhm(np.asarray(times), np.asarray(p)),
                        recruited=int(r)) for p, r in zip(peaks, recruited)]

@@ -544,7 +544,10 @@ int oracle_step(oracle_sim* s, uint64_t ti)
   /* Species_apply_du (species.c:1080-1121) */
   for (uint32_t sp = 0; sp < s->ns; sp++) {
     if (!s->upd[sp]) continue;
-    memcpy(s->u1[sp], s->du[sp], sizeof(double) * s->nv);
+    double* dst = s->u1[sp];
+    const double* src = s->du[sp];
+    #pragma omp parallel for schedule(static)
+    for (uint64_t v = 0; v < s->nv; v++) dst[v] = src[v];
     s->upd[sp] = 0;
   }
   /* Species_apply_fixed_values (species.c:1046-1077) */
@@ -557,9 +560,14 @@ int oracle_step(oracle_sim* s, uint64_t ti)
           s->u1[0][v] = s->init[s->ids[v]][0];
         }
   }
-  for (uint32_t sp = 0; sp < s->ns; sp++)
+  for (uint32_t sp = 0; sp < s->ns; sp++) {
+    int any = 0;
+    for (uint32_t d = 0; d < s->nd; d++) any |= s->fixed[d][sp] != 0;
+    if (!any) continue;
+    #pragma omp parallel for schedule(static)
     for (uint64_t v = 0; v < s->nv; v++)
       if (s->fixed[s->ids[v]][sp]) s->u1[sp][v] = s->init[s->ids[v]][sp];
+  }
   return 0;
 }
 

@@ -130,3 +130,33 @@ def test_shipped_closure_on_unseen_dyads():
          f"({len(held.ca_fine)} samples)")
     assert e_closure < 0.05 * e_plain
     net.close()
+
+
+def test_closure_gating_replayed_as_graphs_equals_direct_launches(trained):
+    """Closure gating inside captured periods (features, prediction and
+    gating per gating step), replayed several times with eager work and
+    synchronisations between the calls, equals direct launches."""
+    net, norm = trained
+    ds = _dyads(range(300, 304))
+    mk = lambda: cs.Ensemble([(d.geometry, d.model, cs.Options(h=12.0, seed=30 + k, open={"ryr": [d.open_index]}))
+                              for k, d in enumerate(ds)], instances=list(range(4)))
+    a, b = mk(), mk()
+    for e in (a, b):
+        e.set_gating(cs.Ensemble.GATE_CLOSURE, net)
+    a.set_graph(True)
+    for rep in range(3):
+        n = 40 + 3 * rep
+        a.step(n)
+        b.step(n)
+        np.testing.assert_array_equal(a.stats()[2], b.stats()[2])
+        a.synchronize()
+        np.testing.assert_array_equal(a.closure_predictions(), b.closure_predictions())
+        for i in range(a.count):
+            for f in range(2):
+                np.testing.assert_array_equal(a.channel_states(i, f), b.channel_states(i, f))
+            for s in range(a.ns):
+                np.testing.assert_array_equal(a.species(i, s), b.species(i, s), err_msg=f"replay {rep} member {i}")
+        _say(f"closure graph replay {rep}: equal after {a.time_ind - 1} steps")
+    np.testing.assert_array_equal(a.monitor(), b.monitor())
+    a.close()
+    b.close()
