@@ -165,9 +165,12 @@ def closure_record(count=256, fine_count=16, steps=60, fine_steps=6):
     import closure
     t0 = time.perf_counter()
     net, norm, trained = closure.load(CLOSURE_PARAMS)
+    flux = closure.load_flux(CLOSURE_PARAMS)
     ds, members = ensemble_members(count, 12.0, (512.0, 1024.0), (25, 150), 5000, "f32")
     ens = cs.Ensemble(members, instances=list(range(count)))
     ens.set_gating(cs.Ensemble.GATE_CLOSURE, net)
+    if flux is not None:
+        ens.set_flux_closure(flux[0])
     _, nv12, _ = ens.info()
     ms12, steps12, rate12 = ensemble_rate(ens, steps, 50.0)
     flagged = int(np.count_nonzero(ens.monitor()))
@@ -180,12 +183,15 @@ def closure_record(count=256, fine_count=16, steps=60, fine_steps=6):
     ms3, steps3, rate3 = ensemble_rate(ens3, fine_steps, 50.0, warmup=2)
     ens3.close()
     net.close()
+    if flux is not None:
+        flux[0].close()
     log(f"plain 3 nm: {fine_count} dyads, {nv3} voxels, {ms3:.4f} ms/step -> {rate3:.5f} realisations/s "
         f"(set up in {time.perf_counter() - t0:.1f}s)")
     return {"workload": "dyads of 512-1024 nm edges, 25-150 RyRs, 0-50% pRyR, Philox, FP32, 50 ms runs",
             "closure": {"h_nm": 12.0, "realisations": count, "voxels": nv12, "ms_per_step": round(ms12, 4),
                         "steps_per_realisation": steps12, "realisations_per_s": round(rate12, 4),
                         "network": "trained" if trained else "untrained (same cost)",
+                        "flux_closure": flux is not None,
                         "range_monitor_flagged": flagged},
             "plain": {"h_nm": 3.0, "realisations": fine_count, "voxels": nv3, "ms_per_step": round(ms3, 4),
                       "steps_per_realisation": steps3, "realisations_per_s": round(rate3, 6)},

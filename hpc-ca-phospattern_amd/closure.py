@@ -12,12 +12,21 @@ channel's fine `to` voxel, which under the reference's refinement of one
 centre sub-voxel pair) is the centre sub-voxel of the coarse one -- and the
 coarse ensemble gates on that prediction (Ensemble.set_gating).
 
+A second network on the same inputs closes the release: the fine grid's
+two-voxel flux (ryr_flux, boundaryfluxes.c:87-141) acts between 3 nm
+voxels at the mouth, which deplete on the SR side and fill on the cleft
+side within a few fine steps, while the coarse grid's acts between 12 nm
+voxels that barely move; the coarse grid then releases several times too
+much calcium per opening.  The flux network predicts, for an open channel,
+the ratio of the fine release to the coarse one over the same time
+(Ensemble.set_flux_closure), and the coarse flux is scaled by it.
+
 Training data come from paired runs of the same 12 nm dyads at 3 nm (fine)
 and 12 nm (coarse): the fine ensemble gates with Ca-independent rates that
 visit many open/closed configurations, and the coarse ensemble replays its
 channel states (GATE_REPLAY), so both grids see the same release schedule;
 each coarse step's features are paired with the fine mouth concentration
-at the same time.  Inputs: the coarse Ca of the 3x3x3 voxels around the
+at the same time, and for the open channels with the release ratio.  Inputs: the coarse Ca of the 3x3x3 voxels around the
 cleft voxel, the SR Ca at the channel's SR voxel, and the open channels
 within 1 and 2 coarse voxels (Ensemble.closure_features); concentrations
 enter as logarithms, everything standardised on the training set, and the
@@ -99,6 +108,34 @@ class PairedSamples:
     owner: np.ndarray      # [n] realisation of the sample
     fine_steps: int
     coarse_steps: int
+    open: Optional[np.ndarray] = None    # [n] channel open for the next step's flux
+    ratio: Optional[np.ndarray] = None   # [n] fine release / coarse release (where open)
+
+
+def release_factors(ens: cs.Ensemble, h: float) -> np.ndarray:
+    """Per channel (channel order), the two-voxel flux per step and unit
+    concentration difference of its flux: (1 - K_0) / 2 of ryr_flux
+    (boundaryfluxes.c:87-141) at the ensemble's dt and voxel size h."""
+    lib = cs.device_lib()
+    out = []
+    for st in ens.setups:
+        p = st.problem
+        for f in range(p.num_fluxes):
+            F = p.fluxes[f]
+            if F.discrete:
+                k = float(lib.cs_ryr_flux(ens.dt, h, 1.0, 0.0, F.params))
+                out.extend([k] * int(F.num_pairs))
+    return np.asarray(out)
+
+
+def release_ratio(fine_from, fine_to, coarse_from, coarse_to, k_fine, k_coarse, steps_per: int,
+                  h_ratio: float) -> np.ndarray:
+    """Fine release over one coarse step (steps_per fine steps of
+    k_fine (from - to) into a voxel of (1/h_ratio)^3 the volume) over the
+    coarse grid's k_coarse (from - to) for the same channel."""
+    d_f = np.asarray(fine_from) - np.asarray(fine_to)
+    d_c = np.maximum(np.asarray(coarse_from) - np.asarray(coarse_to), 1e-3)
+    return np.maximum(d_f, 0.0) * k_fine * steps_per / (d_c * k_coarse * h_ratio ** 3)
 
 
 def paired_samples(dyads: Sequence, seeds: Sequence[int], fine_h: float = 3.0, coarse_h: float = 12.0,
@@ -121,7 +158,8 @@ def paired_samples(dyads: Sequence, seeds: Sequence[int], fine_h: float = 3.0, c
     coarse.set_gating(cs.Ensemble.GATE_REPLAY)
     coarse.copy_channel_states(fine)
     owner = coarse.channel_owner()
-    raw, yf, yc, own = [], [], [], []
+    kf, kc = release_factors(fine, fine_h), release_factors(coarse, coarse_h)
+    raw, yf, yc, own, op, rr = [], [], [], [], [], []
     for k in range(1, coarse_steps + 1):
         fine.step(steps_per)
         coarse.copy_channel_states(fine)
@@ -131,10 +169,13 @@ def paired_samples(dyads: Sequence, seeds: Sequence[int], fine_h: float = 3.0, c
             yf.append(fine.channel_ca())
             yc.append(coarse.channel_ca())
             own.append(owner)
+            op.append(fine.channel_open())
+            rr.append(release_ratio(fine.channel_sr_ca(), yf[-1], coarse.channel_sr_ca(), yc[-1], kf, kc,
+                                    steps_per, coarse_h / fine_h))
     fine.close()
     coarse.close()
     return PairedSamples(np.concatenate(raw), np.concatenate(yf), np.concatenate(yc), np.concatenate(own),
-                         coarse_steps * steps_per, coarse_steps)
+                         coarse_steps * steps_per, coarse_steps, np.concatenate(op), np.concatenate(rr))
 
 
 def train(samples: PairedSamples, params0: np.ndarray, steps: int = 3000, batch: int = 1024, lr: float = 2e-3,
@@ -144,13 +185,48 @@ def train(samples: PairedSamples, params0: np.ndarray, steps: int = 3000, batch:
     norm = fit_normalisation(samples.raw, samples.ca_fine, margin)
     net = cs.Mlp(params0)
     norm.apply(net)
-    net.set_data(norm.inputs(samples.raw), norm.target(samples.ca_fine), seed=seed)
+    _fit(net, norm.inputs(samples.raw), norm.target(samples.ca_fine), steps, batch, lr, seed)
+    return net, norm
+
+
+def _fit(net: cs.Mlp, x: np.ndarray, t: np.ndarray, steps: int, batch: int, lr: float, seed: int):
+    net.set_data(x, t, seed=seed)
     left = steps
     while left > 0:
         k = min(left, 500)
-        net.train(k, min(batch, len(samples.ca_fine)), lr)
+        net.train(k, min(batch, len(t)), lr)
         left -= k
-    return net, norm
+
+
+def flux_selection(samples: PairedSamples) -> np.ndarray:
+    """Samples that train the flux network: open channels with a release."""
+    return (samples.open > 0) & np.isfinite(samples.ratio) & (samples.ratio > 0)
+
+
+def train_flux(samples: PairedSamples, norm: Normalisation, params0: np.ndarray, steps: int = 3000,
+               batch: int = 1024, lr: float = 2e-3, seed: int = 1):
+    """Fit the flux network to the open channels' release ratios, on the
+    gating network's input standardisation (one range monitor serves
+    both); returns the network and its normalisation."""
+    sel = flux_selection(samples)
+    if not sel.any():
+        raise ValueError("no open channel among the samples")
+    t = np.log(samples.ratio[sel])
+    nf = Normalisation(norm.in_mean, norm.in_std, norm.in_lo, norm.in_hi, float(t.mean()),
+                       float(max(t.std(), 1e-3)), norm.margin)
+    net = cs.Mlp(params0)
+    nf.apply(net)
+    _fit(net, nf.inputs(samples.raw[sel]), nf.target(samples.ratio[sel]), steps, batch, lr, seed)
+    return net, nf
+
+
+def flux_errors(net: cs.Mlp, norm: Normalisation, samples: PairedSamples) -> Tuple[float, float]:
+    """Mean squared error of the log release ratio over the open channels:
+    the flux network's, and the plain coarse grid's (ratio 1)."""
+    sel = flux_selection(samples)
+    t = np.log(samples.ratio[sel])
+    pred = np.log(norm.concentration(net.forward(norm.inputs(samples.raw[sel]))))
+    return float(np.mean((pred - t) ** 2)), float(np.mean(t ** 2))
 
 
 def log_errors(net: cs.Mlp, norm: Normalisation, samples: PairedSamples) -> Tuple[float, float]:
@@ -163,11 +239,32 @@ def log_errors(net: cs.Mlp, norm: Normalisation, samples: PairedSamples) -> Tupl
     return e_closure, e_plain
 
 
-def save(path: str, net: cs.Mlp, norm: Normalisation, **meta):
-    """Trained parameters and normalisation as a plain .npz (no pickles)."""
+def save(path: str, net: cs.Mlp, norm: Normalisation, flux=None, **meta):
+    """Trained parameters and normalisation as a plain .npz (no pickles);
+    flux = (network, normalisation) of the flux closure, if trained."""
     extra = {k: np.asarray(v) for k, v in meta.items()}
+    if flux is not None:
+        fnet, fnorm = flux
+        extra["flux_params"] = fnet.params
+        extra["flux_out"] = np.array([fnorm.out_mean, fnorm.out_std, fnorm.margin])
     np.savez(path, params=net.params, in_mean=norm.in_mean, in_std=norm.in_std, in_lo=norm.in_lo, in_hi=norm.in_hi,
              out=np.array([norm.out_mean, norm.out_std, norm.margin]), **extra)
+
+
+def load_flux(path: str):
+    """(network, normalisation) of the flux closure saved with the gating
+    network, or None when the file has none."""
+    import os
+    if not os.path.exists(path):
+        return None
+    z = np.load(path)
+    if "flux_params" not in z.files:
+        return None
+    o = z["flux_out"]
+    norm = Normalisation(z["in_mean"], z["in_std"], z["in_lo"], z["in_hi"], float(o[0]), float(o[1]), float(o[2]))
+    net = cs.Mlp(z["flux_params"])
+    norm.apply(net)
+    return net, norm
 
 
 def load(path: str, init_seed: int = 0):

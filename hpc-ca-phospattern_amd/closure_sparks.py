@@ -8,14 +8,17 @@ ways, each as one ensemble:
            (species.c:938-1035) and gates each channel on its own cleft
            voxel (boundaryfluxes.c:295-359);
   closure  12 nm, every channel gated on the closure network's prediction
-           of the fine mouth Ca (Ensemble.GATE_CLOSURE);
+           of the fine mouth Ca (Ensemble.GATE_CLOSURE) and its flux scaled
+           by the flux network's fine-to-coarse release ratio
+           (Ensemble.set_flux_closure);
+  gating   the same without the flux network (gating closure only);
   coarse   plain method at 12 nm (gating on the coarse cleft voxel).
 
 Every realisation gives peak dF/F0, duration (FDHM) and RyRs recruited
 (sparks.py) from a linescan of the dye through the cluster centre along x,
 sampled at the same times on both grids (one coarse step is 16 fine
-steps).  The closure and coarse distributions are compared with the fine
-one by two-sample Kolmogorov-Smirnov tests.  Realisations whose closure
+steps).  The closure, gating and coarse distributions are compared with
+the fine one by two-sample Kolmogorov-Smirnov tests.  Realisations whose closure
 inputs left the training range at any evaluation (the range monitor) are
 run again with the fine method, and their statistics replace the closure
 ones: the fine-grid fallback.
@@ -94,7 +97,7 @@ class Run:
 
 def run(dyads: Sequence[dyad.Dyad], seeds: Sequence[int], h: float, nsteps: int, save_every: int,
         gating: int = cs.Ensemble.GATE_OWN, net: Optional[cs.Mlp] = None, precision: str = "f32",
-        graphs: bool = True, say=None) -> Run:
+        graphs: bool = True, say=None, flux_net: Optional[cs.Mlp] = None) -> Run:
     """Spark statistics of every dyad at voxel size h, one ensemble (steps
     replayed as captured graphs when `graphs`)."""
     ens = cs.Ensemble([(d.geometry, d.model, cs.Options(h=h, seed=s, open={"ryr": [d.open_index]},
@@ -102,6 +105,8 @@ def run(dyads: Sequence[dyad.Dyad], seeds: Sequence[int], h: float, nsteps: int,
                        for d, s in zip(dyads, seeds)], instances=list(range(len(dyads))))
     if gating != cs.Ensemble.GATE_OWN:
         ens.set_gating(gating, net)
+    if flux_net is not None:
+        ens.set_flux_closure(flux_net)
     if graphs:
         ens.set_graph(True)
     _, nv, _ = ens.info()
@@ -135,16 +140,22 @@ def ks(a: Sequence[sparks.SparkStats], b: Sequence[sparks.SparkStats]) -> dict:
 
 
 def compare(count: int = 256, edge=(192.0, 256.0), ryr=(9, 30), tstop_ms: float = 0.3, save_coarse: int = 20,
-            seed0: int = 1, net: Optional[cs.Mlp] = None, closure_graphs: bool = True, say=print) -> dict:
-    """Fine, closure (with fallback) and coarse statistics of `count`
-    realisations, their KS comparisons and throughputs."""
+            seed0: int = 1, net: Optional[cs.Mlp] = None, flux_net: Optional[cs.Mlp] = None,
+            closure_graphs: bool = True, say=print) -> dict:
+    """Fine, closure (with fallback), gating-only and coarse statistics of
+    `count` realisations, their KS comparisons and throughputs.  Without
+    `net`, both networks come from closure_params.npz."""
     seeds = list(range(seed0, seed0 + count))
     dyads = spark_dyads(seeds, edge, ryr)
     own_net = net is None
+    fnet = flux_net
     if own_net:
-        net, _, trained = closure.load(os.path.join(HERE, "closure_params.npz"))
+        path = os.path.join(HERE, "closure_params.npz")
+        net, _, trained = closure.load(path)
         if not trained:
             raise RuntimeError("closure_params.npz is missing: train the closure first (train_closure.py)")
+        fl = closure.load_flux(path)
+        fnet = fl[0] if fl is not None else None
     probe_f = cs.Simulation(dyads[0].geometry, dyads[0].model, cs.Options(h=FINE_H, seed=1))
     probe_c = cs.Simulation(dyads[0].geometry, dyads[0].model, cs.Options(h=COARSE_H, seed=1))
     ratio = probe_c.dt / probe_f.dt
@@ -160,8 +171,15 @@ def compare(count: int = 256, edge=(192.0, 256.0), ryr=(9, 30), tstop_ms: float 
     res = {}
     res["coarse"] = run(dyads, seeds, COARSE_H, n_coarse, save_coarse, say=say)
     say(f"coarse: {res['coarse'].seconds:.1f} s")
-    res["closure"] = run(dyads, seeds, COARSE_H, n_coarse, save_coarse, cs.Ensemble.GATE_CLOSURE, net,
-                         graphs=closure_graphs, say=say)
+    res["gating"] = run(dyads, seeds, COARSE_H, n_coarse, save_coarse, cs.Ensemble.GATE_CLOSURE, net,
+                        graphs=closure_graphs, say=say)
+    say(f"gating closure only: {res['gating'].seconds:.1f} s")
+    if fnet is not None:
+        res["closure"] = run(dyads, seeds, COARSE_H, n_coarse, save_coarse, cs.Ensemble.GATE_CLOSURE, net,
+                             graphs=closure_graphs, say=say, flux_net=fnet)
+    else:
+        say("no flux network in closure_params.npz: the closure is the gating closure alone")
+        res["closure"] = res["gating"]
     say(f"closure: {res['closure'].seconds:.1f} s, {int(np.count_nonzero(res['closure'].flagged))} realisations "
         "flagged by the range monitor")
     res["fine"] = run(dyads, seeds, FINE_H, n_coarse * r, save_coarse * r, say=say)
@@ -178,17 +196,20 @@ def compare(count: int = 256, edge=(192.0, 256.0), ryr=(9, 30), tstop_ms: float 
             closure_stats[i] = fb.stats[k]
     if own_net:
         net.close()
+        if fnet is not None:
+            fnet.close()
     out = {"count": count, "edge_nm": list(edge), "ryr": list(ryr), "tstop_ms": n_coarse * dt_c,
            "coarse_steps": n_coarse, "fine_steps": n_coarse * r, "kinetics": FAST,
-           "flagged": int(len(flagged))}
-    for name in ("fine", "closure", "coarse"):
+           "flagged": int(len(flagged)), "flux_closure": fnet is not None}
+    for name in ("fine", "closure", "gating", "coarse"):
         rr = res[name]
         sec = rr.seconds + (fallback_s if name == "closure" else 0.0)
         out[name] = {"voxels": rr.voxels, "seconds": round(sec, 3), "realisations_per_s": round(count / sec, 3)}
     out["closure"]["ks_vs_fine"] = ks(closure_stats, res["fine"].stats)
+    out["gating"]["ks_vs_fine"] = ks(res["gating"].stats, res["fine"].stats)
     out["coarse"]["ks_vs_fine"] = ks(res["coarse"].stats, res["fine"].stats)
     out["speedup_closure_vs_fine"] = round(out["closure"]["realisations_per_s"] / out["fine"]["realisations_per_s"], 1)
-    for name in ("closure", "coarse"):
+    for name in ("closure", "gating", "coarse"):
         for k in KEYS:
             q = out[name]["ks_vs_fine"][k]
             say(f"{name:8s} {k:9s}: mean {q['mean']:.4g} vs fine {q['ref_mean']:.4g}, KS D {q['D']:.4f} p {q['p']:.4f}")

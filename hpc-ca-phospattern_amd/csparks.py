@@ -308,6 +308,10 @@ def device_lib() -> C.CDLL:
         lib.cs_batch_closure_features.argtypes = [P, pf]
         lib.cs_batch_channel_ca.argtypes = [P, pd]
         lib.cs_batch_closure_predictions.argtypes = [P, pd]
+        lib.cs_batch_channel_sr_ca.argtypes = [P, pd]
+        lib.cs_batch_channel_open.argtypes = [P, C.POINTER(C.c_uint8)]
+        lib.cs_batch_set_flux_closure.argtypes = [P, P]
+        lib.cs_batch_flux_predictions.argtypes = [P, pd]
         lib.cs_batch_monitor.argtypes = [P, C.POINTER(C.c_uint32)]
         lib.cs_mlp_create.argtypes = [pf, C.POINTER(P)]
         lib.cs_mlp_destroy.argtypes = [P]
@@ -1101,6 +1105,33 @@ class Ensemble:
         out = np.zeros(max(self.num_channels, 1))
         _check(device_lib().cs_batch_closure_predictions(self.handle, out.ctypes.data_as(C.POINTER(C.c_double))),
                "cs_batch_closure_predictions")
+        return out[:self.num_channels]
+
+    def channel_sr_ca(self) -> np.ndarray:
+        """Absolute Ca at every channel's SR (from) voxel."""
+        out = np.zeros(max(self.num_channels, 1))
+        _check(device_lib().cs_batch_channel_sr_ca(self.handle, out.ctypes.data_as(C.POINTER(C.c_double))),
+               "cs_batch_channel_sr_ca")
+        return out[:self.num_channels]
+
+    def channel_open(self) -> np.ndarray:
+        """State of every channel (1 open), in channel order."""
+        out = np.zeros(max(self.num_channels, 1), dtype=np.uint8)
+        _check(device_lib().cs_batch_channel_open(self.handle, out.ctypes.data_as(C.POINTER(C.c_uint8))),
+               "cs_batch_channel_open")
+        return out[:self.num_channels]
+
+    def set_flux_closure(self, net: "Optional[Mlp]"):
+        """With GATE_CLOSURE: scale every channel's flux by the flux
+        network's predicted ratio of fine to coarse release (None: off)."""
+        self._fnet = net                      # the network must outlive its use
+        _check(device_lib().cs_batch_set_flux_closure(self.handle, net.handle if net is not None else None),
+               "cs_batch_set_flux_closure")
+
+    def flux_predictions(self) -> np.ndarray:
+        out = np.zeros(max(self.num_channels, 1))
+        _check(device_lib().cs_batch_flux_predictions(self.handle, out.ctypes.data_as(C.POINTER(C.c_double))),
+               "cs_batch_flux_predictions")
         return out[:self.num_channels]
 
     def monitor(self) -> np.ndarray:

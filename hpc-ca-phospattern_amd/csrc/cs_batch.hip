@@ -161,14 +161,18 @@ k_batch_stencil(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __re
 /* One wavefront per active realisation: fluxes of a step, then gating of
  * the next.  The flux rows, their entry order and skip rules are k_flux's
  * (a row's sum in double, added once); the gating is gate_one on the
- * absolute Ca at each channel's cleft voxel, read after the fluxes. */
+ * absolute Ca at each channel's cleft voxel, read after the fluxes, or on
+ * the closure's prediction (cpred).  fpred, when given, scales each
+ * channel's flux by the flux closure's ratio of the fine-grid release to
+ * the coarse one. */
 template <typename T>
 __global__ void __launch_bounds__(BRIDGE_BATCH_THREADS)
 k_batch_bridge(const T* __restrict__ u, T* __restrict__ un, const DevParams<T>* __restrict__ P,
                const BatchInst* __restrict__ insts, const uint8_t* __restrict__ active,
                const BatchChan* __restrict__ chans, const BRow* __restrict__ rows, const BEnt* __restrict__ ents,
                uint8_t* __restrict__ states, uint8_t* __restrict__ recruited, uint64_t* __restrict__ last_open,
-               BridgeArgs a, const double* __restrict__ uniforms, const double* __restrict__ cpred)
+               BridgeArgs a, const double* __restrict__ uniforms, const double* __restrict__ cpred,
+               const double* __restrict__ fpred)
 {
   const uint32_t i = blockIdx.x;
   if (active && !active[i]) return;
@@ -185,8 +189,9 @@ k_batch_bridge(const T* __restrict__ u, T* __restrict__ un, const DevParams<T>* 
         const uint32_t f = x.flux_sign & 0xFFu;
         if (!((a.fluxmask >> f) & 1u)) continue;
         if (x.state != NO_ROW && !states[x.state]) continue;
-        const double J = flux_j(P->flux[f], (double)u[x.elem_from], (double)u[x.elem_to], x.rest_from,
-                                x.rest_to, dt, h);
+        double J = flux_j(P->flux[f], (double)u[x.elem_from], (double)u[x.elem_to], x.rest_from, x.rest_to,
+                          dt, h);
+        if (fpred && x.state != NO_ROW) J *= fpred[x.state];
         sum += (x.flux_sign >> 8) ? J : -J;
         any = true;
       }
@@ -284,12 +289,22 @@ __global__ void k_channel_ca(const T* __restrict__ u, const BatchChan* __restric
   if (c < nchan) out[c] = (double)u[chans[c].elem_to] + chans[c].rest_to;
 }
 
+/* Absolute Ca at every channel's SR (from) voxel. */
+template <typename T>
+__global__ void k_channel_sr_ca(const T* __restrict__ u, const BatchChanGeo* __restrict__ cgeo, uint32_t nchan,
+                                double* __restrict__ out)
+{
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < nchan) out[c] = (double)u[cgeo[c].elem_from] + cgeo[c].rest_from;
+}
+
 /* The closure's prediction for every channel: 32 channels per wavefront
  * (lane l%32; both lane groups gather half of the inputs each), the
  * network on the matrix cores gives the fine-grid Ca at each channel
  * mouth, which the bridge then gates on instead of the coarse cleft
- * voxel's Ca.  A channel whose normalised inputs leave the training range
- * counts in its realisation's monitor (flags). */
+ * voxel's Ca (or, with the flux closure's network, the ratio of the fine
+ * release to the coarse one).  A channel whose normalised inputs leave the
+ * training range counts in its realisation's monitor (flags, if given). */
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_closure_predict(const T* __restrict__ u, const uint8_t* __restrict__ ids, const DevParams<T>* __restrict__ P,
@@ -323,7 +338,7 @@ k_closure_predict(const T* __restrict__ u, const uint8_t* __restrict__ ids, cons
   const bool oor_any = oor || __shfl_xor((int)oor, 32, 64);
   if (g != 0 || !valid) return;
   if (active && !active[cg.inst]) return;
-  if (oor_any) atomicAdd(&flags[cg.inst], 1u);
+  if (flags && oor_any) atomicAdd(&flags[cg.inst], 1u);
   cpred[c] = exp((double)y * N->out_std + N->out_mean);
 }
 
@@ -518,6 +533,8 @@ struct cs_batch {
   /* closure gating and replay */
   uint32_t gating = CS_GATE_OWN;
   const MlpDev* d_net = nullptr;           /* the closure's network            */
+  const MlpDev* d_fnet = nullptr;          /* the flux closure's network       */
+  double* d_fpred = nullptr;               /* its ratios, per channel          */
   BatchChanGeo* d_cgeo = nullptr;
   uint32_t* d_nb_start = nullptr;
   uint32_t* d_nb = nullptr;
@@ -561,7 +578,7 @@ void free_batch(cs_batch* b)
   for (auto g : b->graph_defs) (void)hipGraphDestroy(g);
   void* ptrs[] = {b->d_u[0], b->d_u[1], b->d_ids, b->d_params, b->d_geo, b->d_tiles, b->d_queue, b->d_inst,
                   b->d_chan, b->d_rows, b->d_ents, b->d_states, b->d_recruited, b->d_active, b->d_last_open,
-                  b->d_cgeo, b->d_nb_start, b->d_nb, b->d_cpred, b->d_chca, b->d_feat, b->d_flags,
+                  b->d_cgeo, b->d_nb_start, b->d_nb, b->d_cpred, b->d_fpred, b->d_chca, b->d_feat, b->d_flags,
                   b->d_uniforms, b->d_stats, b->d_lw, b->d_lpart, b->d_lproj, b->d_loff, b->d_ti};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -699,11 +716,28 @@ bool b_gates(const cs_batch* b, uint64_t ti) { return b->ndiscrete && ti % b->mo
 
 template <typename T>
 void launch_bbridge(cs_batch* b, const void* u, void* un, const BridgeArgs& a, const double* d_uni,
-                    const double* cpred = nullptr)
+                    const double* cpred = nullptr, const double* fpred = nullptr)
 {
   hipLaunchKernelGGL(k_batch_bridge<T>, dim3(b->count), dim3(BRIDGE_BATCH_THREADS), 0, b->stream, (const T*)u,
                      (T*)un, (const DevParams<T>*)b->d_params, b->d_inst, b->d_active, b->d_chan, b->d_rows,
-                     b->d_ents, b->d_states, b->d_recruited, b->d_last_open, a, d_uni, cpred);
+                     b->d_ents, b->d_states, b->d_recruited, b->d_last_open, a, d_uni, cpred, fpred);
+}
+
+/* The closure networks' predictions from the state u: the gating
+ * network's mouth Ca (with the range monitor) and, when set, the flux
+ * network's release ratios. */
+template <typename T>
+void launch_predict(cs_batch* b, const void* u)
+{
+  const unsigned blocks = (b->nchan_total + 127) / 128;
+  if (!blocks) return;
+  hipLaunchKernelGGL(k_closure_predict<T>, dim3(blocks), dim3(256), 0, b->stream, (const T*)u, b->d_ids,
+                     (const DevParams<T>*)b->d_params, b->d_net, b->d_geo, b->d_active, b->d_cgeo, b->d_states,
+                     b->d_nb_start, b->d_nb, b->nchan_total, b->d_cpred, b->d_flags);
+  if (b->d_fnet)
+    hipLaunchKernelGGL(k_closure_predict<T>, dim3(blocks), dim3(256), 0, b->stream, (const T*)u, b->d_ids,
+                       (const DevParams<T>*)b->d_params, b->d_fnet, b->d_geo, b->d_active, b->d_cgeo, b->d_states,
+                       b->d_nb_start, b->d_nb, b->nchan_total, b->d_fpred, (uint32_t*)nullptr);
 }
 
 /* Steps [first, first + n) on the stream: gating of the first step, then
@@ -726,21 +760,22 @@ int b_run(cs_batch* b, uint64_t first, uint64_t n, const double* d_uni, hipEvent
     a.uni_base = d_uni ? gate_idx * b->uni_stride : 0;
     a.D = D;
     if (a.gate) gate_idx++;
-    if (b->gating == CS_GATE_CLOSURE && a.gate) {
-      /* fluxes first, then the closure's prediction from the new state,
-       * then gating on it */
+    const bool fclosure = b->gating == CS_GATE_CLOSURE && b->d_fnet;
+    if (b->gating == CS_GATE_CLOSURE && (a.gate || fclosure)) {
+      /* fluxes first (scaled by the flux network's ratios, predicted from
+       * the state they read), then the predictions from the new state,
+       * then gating on them; with the flux network every bridge predicts,
+       * since the next step's fluxes read its ratios */
       if (a.fluxmask) {
         BridgeArgs f = a;
         f.gate = 0;
-        launch_bbridge<T>(b, u, un, f, d_uni);
+        launch_bbridge<T>(b, u, un, f, d_uni, nullptr, fclosure ? b->d_fpred : nullptr);
       }
-      const unsigned blocks = (b->nchan_total + 127) / 128;
-      if (blocks)
-        hipLaunchKernelGGL(k_closure_predict<T>, dim3(blocks), dim3(256), 0, b->stream, (const T*)un, b->d_ids,
-                           (const DevParams<T>*)b->d_params, b->d_net, b->d_geo, b->d_active, b->d_cgeo, b->d_states,
-                           b->d_nb_start, b->d_nb, b->nchan_total, b->d_cpred, b->d_flags);
-      a.fluxmask = 0;
-      launch_bbridge<T>(b, un, un, a, d_uni, b->d_cpred);
+      launch_predict<T>(b, un);
+      if (a.gate) {
+        a.fluxmask = 0;
+        launch_bbridge<T>(b, un, un, a, d_uni, b->d_cpred);
+      }
       return;
     }
     if (a.fluxmask || a.gate) launch_bbridge<T>(b, u, un, a, d_uni);
@@ -1108,6 +1143,7 @@ int cs_batch_create(const cs_problem* problems, uint32_t count, cs_batch** out)
   if (hipMalloc(&b->d_last_open, sizeof(uint64_t) * count) != hipSuccess ||
       hipMalloc(&b->d_flags, sizeof(uint32_t) * count) != hipSuccess ||
       hipMalloc(&b->d_cpred, sizeof(double) * std::max<size_t>(chans.size(), 1)) != hipSuccess ||
+      hipMalloc(&b->d_fpred, sizeof(double) * std::max<size_t>(chans.size(), 1)) != hipSuccess ||
       hipMalloc(&b->d_queue, sizeof(uint32_t) * (NQUEUE + 1)) != hipSuccess ||
       hipMalloc(&b->d_ti, sizeof(uint64_t)) != hipSuccess ||
       hipMalloc(&b->d_stats, sizeof(double) * count * MAXS * MAXD * 3) != hipSuccess)
@@ -1511,6 +1547,14 @@ int cs_batch_set_gating(cs_batch* b, uint32_t mode, const cs_mlp* net)
   return CS_OK;
 }
 
+int cs_batch_set_flux_closure(cs_batch* b, const cs_mlp* net)
+{
+  if (!b) return set_err(CS_EINVAL, "ensemble is NULL");
+  b->d_fnet = net ? (const MlpDev*)cs_internal_mlp_device(net) : nullptr;
+  drop_graphs(b);
+  return CS_OK;
+}
+
 int cs_batch_copy_channel_states(cs_batch* dst, const cs_batch* src)
 {
   if (!dst || !src) return set_err(CS_EINVAL, "NULL argument");
@@ -1573,6 +1617,45 @@ int cs_batch_channel_ca(cs_batch* b, double* out)
                        (const float*)b->d_u[b->cur], b->d_chan, n, b->d_chca);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(out, b->d_chca, sizeof(double) * n, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return CS_OK;
+}
+
+int cs_batch_channel_sr_ca(cs_batch* b, double* out)
+{
+  if (!b || !out) return set_err(CS_EINVAL, "NULL argument");
+  HIP_TRY(hipSetDevice(b->device));
+  const uint32_t n = b->nchan_total;
+  if (!n) return CS_OK;
+  if (!b->d_chca) HIP_TRY(hipMalloc(&b->d_chca, sizeof(double) * n));
+  if (b->esize == 8)
+    hipLaunchKernelGGL(k_channel_sr_ca<double>, dim3((n + 255) / 256), dim3(256), 0, b->stream,
+                       (const double*)b->d_u[b->cur], b->d_cgeo, n, b->d_chca);
+  else
+    hipLaunchKernelGGL(k_channel_sr_ca<float>, dim3((n + 255) / 256), dim3(256), 0, b->stream,
+                       (const float*)b->d_u[b->cur], b->d_cgeo, n, b->d_chca);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, b->d_chca, sizeof(double) * n, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return CS_OK;
+}
+
+int cs_batch_channel_open(cs_batch* b, uint8_t* out)
+{
+  if (!b || !out) return set_err(CS_EINVAL, "NULL argument");
+  HIP_TRY(hipSetDevice(b->device));
+  if (b->nchan_total)
+    HIP_TRY(hipMemcpyAsync(out, b->d_states, b->nchan_total, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return CS_OK;
+}
+
+int cs_batch_flux_predictions(cs_batch* b, double* out)
+{
+  if (!b || !out) return set_err(CS_EINVAL, "NULL argument");
+  HIP_TRY(hipSetDevice(b->device));
+  if (b->nchan_total)
+    HIP_TRY(hipMemcpyAsync(out, b->d_fpred, sizeof(double) * b->nchan_total, hipMemcpyDeviceToHost, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   return CS_OK;
 }

@@ -1,13 +1,15 @@
 #!/usr/bin/env python3
-"""Train the closure network on paired 3 nm / 12 nm runs of generated
-12 nm dyads and write its parameters (closure.save).
+"""Train the closure networks on paired 3 nm / 12 nm runs of generated
+12 nm dyads and write their parameters (closure.save): the gating network
+(fine Ca at each channel mouth) and the flux network (fine-to-coarse
+release ratio of each open channel).
 
     python train_closure.py [out.npz] [--dyads N] [--coarse-steps K] [--steps S]
 
 The fine ensemble gates with Ca-independent rates (closure.schedule_model)
 so that the samples cover many open/closed configurations; the coarse
-ensemble replays its channel states.  Held-out dyads give the log-Ca error
-of the closure and of the plain coarse grid, printed at the end."""
+ensemble replays its channel states.  Held-out dyads give the errors of
+both networks and of the plain coarse grid, printed at the end."""
 import argparse
 import os
 import sys
@@ -43,13 +45,17 @@ def main():
           f"({train.fine_steps} fine / {train.coarse_steps} coarse steps) in {time.time() - t0:.1f}s", flush=True)
     t1 = time.time()
     net, norm = closure.train(train, mlp_init.params(0), steps=args.steps, batch=2048, lr=2e-3)
-    print(f"trained {args.steps} Adam steps in {time.time() - t1:.1f}s", flush=True)
+    fnet, fnorm = closure.train_flux(train, norm, mlp_init.params(1), steps=args.steps, batch=2048, lr=2e-3)
+    print(f"trained 2 x {args.steps} Adam steps in {time.time() - t1:.1f}s "
+          f"({int(closure.flux_selection(train).sum())} open-channel samples for the flux network)", flush=True)
     held = closure.paired_samples(dyads(range(10000, 10012), args.edge, args.ryr), list(range(10000, 10012)),
                                   coarse_steps=args.coarse_steps // 2, sample_every=4)
     e_c, e_p = closure.log_errors(net, norm, held)
     print(f"held-out log-Ca MSE at the channel mouth: closure {e_c:.5f}, plain coarse grid {e_p:.5f}", flush=True)
-    closure.save(args.out, net, norm, held_out_mse=np.array([e_c, e_p]),
-                 samples=np.array([len(train.ca_fine)]))
+    f_c, f_p = closure.flux_errors(fnet, fnorm, held)
+    print(f"held-out MSE of the log release ratio: flux closure {f_c:.5f}, plain coarse grid {f_p:.5f}", flush=True)
+    closure.save(args.out, net, norm, flux=(fnet, fnorm), held_out_mse=np.array([e_c, e_p]),
+                 flux_held_out_mse=np.array([f_c, f_p]), samples=np.array([len(train.ca_fine)]))
     print(f"wrote {args.out}", flush=True)
 
 

@@ -338,8 +338,22 @@ int cs_batch_channel_count(const cs_batch* b, uint32_t* nchan);
 int cs_batch_closure_features(cs_batch* b, float* features);
 /* Absolute Ca at every channel's cleft voxel, [nchan]. */
 int cs_batch_channel_ca(cs_batch* b, double* ca);
+/* Absolute Ca at every channel's SR (from) voxel, [nchan]. */
+int cs_batch_channel_sr_ca(cs_batch* b, double* ca);
+/* State of every channel (1 open), [nchan], in channel order. */
+int cs_batch_channel_open(cs_batch* b, uint8_t* open);
 /* The closure's latest prediction for every channel mouth, [nchan]. */
 int cs_batch_closure_predictions(cs_batch* b, double* ca);
+/* Flux closure, used with CS_GATE_CLOSURE: a second network, on the same
+ * inputs, predicts for every channel the ratio of the release the fine
+ * grid makes through it to the release of the coarse grid's two-voxel
+ * formula (ryr_flux, boundaryfluxes.c:87-141), and every channel's flux is
+ * scaled by it.  Every step then predicts both networks from the state the
+ * next step's fluxes and gating read.  NULL turns it off.  The network
+ * must outlive the ensemble's use of it.  No reference counterpart. */
+int cs_batch_set_flux_closure(cs_batch* b, const struct cs_mlp* net);
+/* The flux closure's latest ratios, [nchan]. */
+int cs_batch_flux_predictions(cs_batch* b, double* ratio);
 /* Per realisation: channel evaluations whose normalised closure inputs
  * left the training range since cs_batch_reset (the range monitor). */
 int cs_batch_monitor(cs_batch* b, uint32_t* flags);

@@ -1,7 +1,8 @@
 """The shipped closure parameters (hpc-ca-phospattern_amd/closure_params.npz,
 written by train_closure.py): a plain .npz that loads without pickles,
-holds a network of the C ABI's size with a usable normalisation, and
-records a held-out error far below the plain coarse grid's."""
+holds the gating and flux networks of the C ABI's size with usable
+normalisations, and records held-out errors far below the plain coarse
+grid's."""
 import os
 
 import numpy as np
@@ -33,6 +34,18 @@ def test_shapes_and_values():
 def test_recorded_held_out_error():
     e_closure, e_plain = _load()["held_out_mse"]
     assert e_closure < 0.05 * e_plain
+
+
+def test_flux_network_is_saved_with_the_gating_network():
+    z = _load()
+    assert z["flux_params"].shape == (mo.N_PARAMS,)
+    assert np.all(np.isfinite(z["flux_params"]))
+    mean, std, margin = z["flux_out"]
+    assert std > 0 and margin >= 0
+    # release ratios below 1 on average: the fine grid releases less
+    assert mean < 0
+    f_closure, f_plain = z["flux_held_out_mse"]
+    assert f_closure < 0.05 * f_plain
 
 
 def test_oracle_forward_is_finite_over_the_training_range():

@@ -68,10 +68,23 @@ def test_replay_copies_schedules_and_features_read_the_fields():
 
 
 @pytest.fixture(scope="module")
-def trained():
+def paired():
     train = closure.paired_samples(_dyads(range(12)), list(range(12)), coarse_steps=300, sample_every=2)
-    _say(f"training set: {len(train.ca_fine)} samples from {train.fine_steps} fine / {train.coarse_steps} coarse steps")
-    net, norm = closure.train(train, mo.init_params(0), steps=2500, batch=1024, lr=2e-3)
+    _say(f"training set: {len(train.ca_fine)} samples from {train.fine_steps} fine / {train.coarse_steps} coarse steps, "
+         f"{int(closure.flux_selection(train).sum())} with the channel open")
+    return train
+
+
+@pytest.fixture(scope="module")
+def trained(paired):
+    net, norm = closure.train(paired, mo.init_params(0), steps=2500, batch=1024, lr=2e-3)
+    yield net, norm
+    net.close()
+
+
+@pytest.fixture(scope="module")
+def trained_flux(paired, trained):
+    net, norm = closure.train_flux(paired, trained[1], mo.init_params(1), steps=2500, batch=1024, lr=2e-3)
     yield net, norm
     net.close()
 
@@ -160,3 +173,77 @@ def test_closure_gating_replayed_as_graphs_equals_direct_launches(trained):
     np.testing.assert_array_equal(a.monitor(), b.monitor())
     a.close()
     b.close()
+
+
+def _constant_net(value):
+    """A network whose output is value for every input: zero weights, so
+    every hidden unit is GELU(0) = 0 and the output is exp(out_mean)."""
+    net = cs.Mlp(np.zeros(cs.MLP_NUM_PARAMS, np.float32))
+    net.set_normalisation(np.zeros(30, np.float32), np.ones(30, np.float32), np.full(30, -np.inf, np.float32),
+                          np.full(30, np.inf, np.float32), float(np.log(value)), 1.0, 0.0)
+    return net
+
+
+def test_flux_closure_scales_each_channel_release(trained):
+    """One step from the same state with the same gating: release ratios
+    1, 1/2 and 1/4 change the fields only at the channels' voxels, by J,
+    J/2 and J/4 (J the coarse two-voxel flux); ratio 1 equals no flux
+    network bit for bit."""
+    net, _ = trained
+    ds = _dyads(range(400, 404))
+    runs = {}
+    for name, ratio in (("none", None), ("r1", 1.0), ("r2", 0.5), ("r4", 0.25)):
+        e = cs.Ensemble([(d.geometry, d.model, cs.Options(h=12.0, seed=40 + k, open={"ryr": [d.open_index]}))
+                         for k, d in enumerate(ds)], instances=list(range(4)))
+        e.set_gating(cs.Ensemble.GATE_CLOSURE, net)
+        fnet = _constant_net(ratio) if ratio else None
+        if fnet is not None:
+            e.set_flux_closure(fnet)
+        e.step(1)
+        runs[name] = (e, fnet)
+    e1 = runs["r1"][0]
+    np.testing.assert_allclose(e1.flux_predictions(), 1.0, rtol=0, atol=0)
+    moved = 0
+    for i in range(len(ds)):
+        for f in range(2):
+            np.testing.assert_array_equal(e1.channel_states(i, f), runs["none"][0].channel_states(i, f))
+        for s in range(e1.ns):
+            a = runs["none"][0].species(i, s)
+            np.testing.assert_array_equal(e1.species(i, s), a, err_msg=f"member {i} species {s}")
+        a, b, q = (runs[k][0].species(i, 0) for k in ("r1", "r2", "r4"))
+        d1, d2 = a - b, b - q
+        np.testing.assert_allclose(d1, 2 * d2, rtol=1e-4, atol=1e-6 * float(np.max(np.abs(d1))))
+        moved += int(np.count_nonzero(np.abs(d1) > 1e-3))
+    assert moved > 0, "no channel released calcium"
+    for e, fnet in runs.values():
+        e.close()
+        if fnet is not None:
+            fnet.close()
+
+
+def test_flux_predictions_are_the_network_on_the_features(trained, trained_flux):
+    net, _ = trained
+    fnet, fnorm = trained_flux
+    ds = _dyads(range(500, 504))
+    ens = cs.Ensemble([(d.geometry, d.model, cs.Options(h=12.0, seed=50 + k, open={"ryr": [d.open_index]}))
+                       for k, d in enumerate(ds)], instances=list(range(4)))
+    ens.set_gating(cs.Ensemble.GATE_CLOSURE, net)
+    ens.set_flux_closure(fnet)
+    ens.step(30)
+    # every step ends with both networks evaluated on the state it leaves
+    feats = ens.closure_features()
+    got = ens.flux_predictions()
+    want = fnorm.concentration(fnet.forward(fnorm.inputs(feats)))
+    assert np.all(np.isfinite(got)) and np.all(got > 0)
+    np.testing.assert_allclose(got, want, rtol=0.03)
+    ens.close()
+
+
+def test_trained_flux_closure_beats_the_plain_coarse_release(trained, trained_flux):
+    fnet, fnorm = trained_flux
+    held = closure.paired_samples(_dyads(range(100, 106)), list(range(100, 106)), coarse_steps=200, sample_every=4)
+    f_c, f_p = closure.flux_errors(fnet, fnorm, held)
+    sel = closure.flux_selection(held)
+    _say(f"held-out MSE of the log release ratio over {int(sel.sum())} open-channel samples: flux closure "
+         f"{f_c:.4f}, plain coarse {f_p:.4f} (mean ratio {np.mean(held.ratio[sel]):.3f})")
+    assert f_c < 0.2 * f_p
