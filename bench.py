@@ -56,7 +56,6 @@ TSTOP_MS = 30.0
 PROFILE_TRAFFIC = os.path.join(REPO, "profiles", "traffic.json")
 WORKLOAD_KEY = "cluster_dyad"
 CLOSURE_PARAMS = os.path.join(REPO, "hpc-ca-phospattern_amd", "closure_params.npz")
-ENSEMBLE_LAUNCH = {"rows": 8, "zchunk": 8, "prefetch": 1, "nontemporal": 0}
 LAUNCH_KEYS = ["rows", "zchunk", "xcd_remap", "prefetch", "nontemporal", "lds_tile", "linear", "yblock",
                "fuse", "fuse_zchunk", "fuse_xcd", "fuse_fixed"]
 
@@ -85,15 +84,18 @@ def cpu_baseline(d, budget_s=15.0):
     return nv * steps / el, cores, steps, el
 
 
+TRAFFIC_SOURCES = ("csparks.hip", "cs_device.h", "cs_physics.h")
+
+
 def kernel_source_hash() -> str:
-    """SHA-256 of the HIP sources the library was built from, which the
-    committed traffic profile records: a profile of other sources is stale."""
+    """SHA-256 of the HIP sources of the single-domain kernels (k_stencil,
+    k_step2 and what they include), which the committed traffic profile
+    records: a profile of other sources is stale."""
     h = hashlib.sha256()
     csrc = os.path.join(REPO, "hpc-ca-phospattern_amd", "csrc")
-    for name in sorted(os.listdir(csrc)):
-        if name.endswith((".hip", ".h")):
-            with open(os.path.join(csrc, name), "rb") as f:
-                h.update(name.encode() + b"\0" + f.read())
+    for name in TRAFFIC_SOURCES:
+        with open(os.path.join(csrc, name), "rb") as f:
+            h.update(name.encode() + b"\0" + f.read())
     return h.hexdigest()
 
 
@@ -126,9 +128,19 @@ def ensemble_members(count, h, edge, ryr, seed0, precision, kinetics=None):
                 for k, d in enumerate(ds)]
 
 
+def tuned(ens):
+    """The ensemble's autotuned launch shape, with its step periods replayed
+    as captured graphs; returns the shape."""
+    best, _ = ens.autotune(2)
+    ens.set_launch(**best)
+    ens.set_graph(True)
+    return best
+
+
 def ensemble_rate(ens, steps, tstop_ms, warmup=10):
     """Measured step time of an ensemble and its realisations/s over runs
-    of tstop_ms."""
+    of tstop_ms.  With graphs, warmup and steps are whole periods, so the
+    warmup captures the graphs and the timed steps only replay them."""
     ens.step(warmup)
     ens.synchronize()
     t1 = time.perf_counter()
@@ -145,23 +157,29 @@ def ensemble_record(count=512, steps=100):
     ds, members = ensemble_members(count, 6.0, (256.0, 512.0), (9, 100), 1, "f32")
     ens = cs.Ensemble(members, instances=list(range(count)))
     _, nv, nbytes = ens.info()
-    ens.set_launch(**ENSEMBLE_LAUNCH)
-    log(f"ensemble: {count} dyads, {nv} voxels, set up in {time.perf_counter() - t0:.1f}s")
-    ms, run_steps, rate = ensemble_rate(ens, steps, 20.0)
+    launch = tuned(ens)
+    P = ens.period
+    log(f"ensemble: {count} dyads, {nv} voxels, set up and tuned in {time.perf_counter() - t0:.1f}s: {launch}, "
+        f"graph period {P}")
+    ms, run_steps, rate = ensemble_rate(ens, P * max(1, steps // P), 20.0, warmup=P)
+    ens.set_graph(False)
     tot_ms, stencil_ms = ens.time_steps(steps)
     ens.close()
     gbs = nv * BYTES_F32 / (stencil_ms * 1e-3) / 1e9
     log(f"ensemble: {ms:.4f} ms/step (stencil {stencil_ms * 1e3:.1f} us, {gbs:.0f} GB/s) -> {rate:.3f} realisations/s")
     return {"workload": f"{count} dyads, edges 256-512 nm at 6 nm, 9-100 RyRs, 0-50% pRyR, Philox, FP32, 20 ms runs",
             "realisations": count, "voxels": nv, "ms_per_step": round(ms, 4), "steps_per_realisation": run_steps,
-            "realisations_per_s": round(rate, 4), "launch": ENSEMBLE_LAUNCH,
+            "realisations_per_s": round(rate, 4), "launch": launch, "graphs": True,
             "stencil": {"kernel": "k_batch_stencil", "kernel_ms": round(stencil_ms, 5),
                         "achieved": round(gbs, 1), "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4)}}
 
 
-def closure_record(count=256, fine_count=16, steps=60, fine_steps=6):
-    """Config-4 comparison: the 12 nm closure ensemble against the plain
-    3 nm ensemble of the same dyads (edges 512-1024 nm, 25-150 RyRs)."""
+def closure_record(count=4096, fine_count=16, steps=120, fine_steps=40, ks=True):
+    """Config-4 comparison: the 12 nm closure ensemble (both networks)
+    against the plain 3 nm ensemble of the same dyads (edges 512-1024 nm,
+    25-150 RyRs), step rates at that size; then, with ks, the spark
+    statistics of the closure against the 3 nm method on 256 small dyads
+    (closure_sparks.compare), the KS test the metric is quoted at."""
     import closure
     t0 = time.perf_counter()
     net, norm, trained = closure.load(CLOSURE_PARAMS)
@@ -172,7 +190,10 @@ def closure_record(count=256, fine_count=16, steps=60, fine_steps=6):
     if flux is not None:
         ens.set_flux_closure(flux[0])
     _, nv12, _ = ens.info()
-    ms12, steps12, rate12 = ensemble_rate(ens, steps, 50.0)
+    launch12 = tuned(ens)
+    P = ens.period
+    log(f"closure 12 nm: set up and tuned in {time.perf_counter() - t0:.1f}s: {launch12}, graph period {P}")
+    ms12, steps12, rate12 = ensemble_rate(ens, P * max(1, steps // P), 50.0, warmup=P)
     flagged = int(np.count_nonzero(ens.monitor()))
     ens.close()
     log(f"closure 12 nm: {count} dyads, {nv12} voxels, {ms12:.4f} ms/step -> {rate12:.3f} realisations/s "
@@ -180,22 +201,40 @@ def closure_record(count=256, fine_count=16, steps=60, fine_steps=6):
     fine = [(g, m, cs.Options(h=3.0, seed=o.seed, open=o.open, precision="f32")) for g, m, o in members[:fine_count]]
     ens3 = cs.Ensemble(fine, instances=list(range(fine_count)))
     _, nv3, _ = ens3.info()
-    ms3, steps3, rate3 = ensemble_rate(ens3, fine_steps, 50.0, warmup=2)
+    launch3 = tuned(ens3)
+    P3 = ens3.period
+    ms3, steps3, rate3 = ensemble_rate(ens3, P3 * max(1, fine_steps // P3), 50.0, warmup=P3)
     ens3.close()
     net.close()
     if flux is not None:
         flux[0].close()
     log(f"plain 3 nm: {fine_count} dyads, {nv3} voxels, {ms3:.4f} ms/step -> {rate3:.5f} realisations/s "
         f"(set up in {time.perf_counter() - t0:.1f}s)")
-    return {"workload": "dyads of 512-1024 nm edges, 25-150 RyRs, 0-50% pRyR, Philox, FP32, 50 ms runs",
+    rec = {"workload": "dyads of 512-1024 nm edges, 25-150 RyRs, 0-50% pRyR, Philox, FP32, 50 ms runs",
             "closure": {"h_nm": 12.0, "realisations": count, "voxels": nv12, "ms_per_step": round(ms12, 4),
                         "steps_per_realisation": steps12, "realisations_per_s": round(rate12, 4),
                         "network": "trained" if trained else "untrained (same cost)",
                         "flux_closure": flux is not None,
-                        "range_monitor_flagged": flagged},
+                        "range_monitor_flagged": flagged, "launch": launch12, "graphs": True},
             "plain": {"h_nm": 3.0, "realisations": fine_count, "voxels": nv3, "ms_per_step": round(ms3, 4),
-                      "steps_per_realisation": steps3, "realisations_per_s": round(rate3, 6)},
+                      "steps_per_realisation": steps3, "realisations_per_s": round(rate3, 6), "launch": launch3,
+                      "graphs": True},
             "speedup": round(rate12 / rate3, 1)}
+    if ks:
+        import closure_sparks
+        t1 = time.perf_counter()
+        out = closure_sparks.compare(count=256, edge=(192.0, 256.0), ryr=(9, 30), tstop_ms=0.3, say=log)
+        rec["ks"] = {"workload": "256 dyads of 192-256 nm edges, 9-30 RyRs, fast spark kinetics, 0.3 ms: 12 nm "
+                                 "closure vs the plain 3 nm method on the same dyads (two-sample KS)",
+                     "closure_vs_fine": {k: {"D": round(v["D"], 4), "p": round(v["p"], 4)}
+                                         for k, v in out["closure"]["ks_vs_fine"].items()},
+                     "plain_12nm_vs_fine": {k: {"D": round(v["D"], 4), "p": round(v["p"], 6)}
+                                            for k, v in out["coarse"]["ks_vs_fine"].items()},
+                     "all_p_above_0.05": all(v["p"] > 0.05 for v in out["closure"]["ks_vs_fine"].values()),
+                     "flagged_to_fine_fallback": out["flagged"],
+                     "speedup_at_this_workload": out["speedup_closure_vs_fine"],
+                     "seconds": round(time.perf_counter() - t1, 1)}
+    return rec
 
 
 def main():
@@ -212,6 +251,7 @@ def main():
     ap.add_argument("--launch", type=str, default="",
                     help="fixed launch shape " + ",".join(LAUNCH_KEYS) + " (a prefix of them; skips autotune)")
     ap.add_argument("--no-extra", action="store_true", help="only the headline dyad (no fp64, ensemble, closure)")
+    ap.add_argument("--no-ks", action="store_true", help="skip the closure's spark-statistics KS comparison")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB (for scaling traffic counters) and exit")
     args = ap.parse_args()
@@ -315,7 +355,7 @@ def main():
     if not args.no_extra:
         out["fp64"] = fp64_record(args.size, 20)
         out["ensemble"] = ensemble_record()
-        out["closure"] = closure_record()
+        out["closure"] = closure_record(ks=not args.no_ks)
     print(json.dumps(out))
 
 

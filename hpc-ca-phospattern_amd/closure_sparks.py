@@ -107,6 +107,8 @@ def run(dyads: Sequence[dyad.Dyad], seeds: Sequence[int], h: float, nsteps: int,
         ens.set_gating(gating, net)
     if flux_net is not None:
         ens.set_flux_closure(flux_net)
+    best, _ = ens.autotune(2)            # the stencil pass's launch shape for these grids
+    ens.set_launch(**best)
     if graphs:
         ens.set_graph(True)
     _, nv, _ = ens.info()

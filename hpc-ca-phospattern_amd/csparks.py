@@ -300,6 +300,7 @@ def device_lib() -> C.CDLL:
         lib.cs_batch_set_launch.argtypes = [P, C.POINTER(CBatchLaunch)]
         lib.cs_batch_autotune.argtypes = [P, C.c_uint64, C.c_uint32, C.POINTER(CBatchLaunch), pd]
         lib.cs_batch_set_graph.argtypes = [P, C.c_uint32]
+        lib.cs_batch_step_period.argtypes = [P, C.POINTER(C.c_uint64)]
         lib.cs_batch_time_steps.argtypes = [P, C.c_uint64, C.c_uint64, pd, pd]
         pf = C.POINTER(C.c_float)
         lib.cs_batch_set_gating.argtypes = [P, C.c_uint32, P]
@@ -1067,6 +1068,13 @@ class Ensemble:
 
     def set_graph(self, on: bool):
         _check(device_lib().cs_batch_set_graph(self.handle, 1 if on else 0), "cs_batch_set_graph")
+
+    @property
+    def period(self) -> int:
+        """Steps of one captured graph period."""
+        p = C.c_uint64()
+        _check(device_lib().cs_batch_step_period(self.handle, C.byref(p)), "cs_batch_step_period")
+        return int(p.value)
 
     GATE_OWN, GATE_REPLAY, GATE_CLOSURE = 0, 1, 2
 

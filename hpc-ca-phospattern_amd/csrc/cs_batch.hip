@@ -111,13 +111,19 @@ struct BridgeArgs {
   uint32_t pad;
 };
 
-template <typename T, int NS, int PF, bool NT>
+/* FM: the step's diffusing species fixed at compile time (with
+ * species-major reactions), or BS_DYN for the mask and reaction mode taken
+ * at run time. */
+constexpr uint32_t BS_DYN = 0xFFFFFFFFu;
+
+template <typename T, int NS, int PF, bool NT, uint32_t FM = BS_DYN>
 __global__ void __launch_bounds__(BX * 8)
 k_batch_stencil(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__ ids,
                 const DevParams<T>* __restrict__ P, const BatchGeo* __restrict__ geos,
-                const BatchTile* __restrict__ tiles, uint32_t ntiles, uint32_t dmask, uint32_t react,
+                const BatchTile* __restrict__ tiles, uint32_t ntiles, uint32_t dmask_rt, uint32_t react_rt,
                 uint32_t* __restrict__ queue /* [NQUEUE + 1], zero between launches */)
 {
+  const uint32_t dmask = FM != BS_DYN ? FM : dmask_rt, react = FM != BS_DYN ? 2u : react_rt;
   __shared__ BlockTables<T, NS> S;
   __shared__ uint32_t s_tile;
   const int nthr = BX * blockDim.y;
@@ -298,20 +304,22 @@ __global__ void k_channel_sr_ca(const T* __restrict__ u, const BatchChanGeo* __r
   if (c < nchan) out[c] = (double)u[cgeo[c].elem_from] + cgeo[c].rest_from;
 }
 
-/* The closure's prediction for every channel: 32 channels per wavefront
- * (lane l%32; both lane groups gather half of the inputs each), the
- * network on the matrix cores gives the fine-grid Ca at each channel
- * mouth, which the bridge then gates on instead of the coarse cleft
- * voxel's Ca (or, with the flux closure's network, the ratio of the fine
- * release to the coarse one).  A channel whose normalised inputs leave the
- * training range counts in its realisation's monitor (flags, if given). */
+/* The closure's predictions for every channel: 32 channels per wavefront
+ * (lane l%32; both lane groups gather half of the inputs each, once), the
+ * gating network N on the matrix cores gives the fine-grid Ca at each
+ * channel mouth, which the bridge then gates on instead of the coarse
+ * cleft voxel's Ca, and the flux network F (if given), on the same raw
+ * inputs, the ratio of the fine release to the coarse one.  A channel
+ * whose inputs, normalised for N, leave N's training range counts in its
+ * realisation's monitor. */
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_closure_predict(const T* __restrict__ u, const uint8_t* __restrict__ ids, const DevParams<T>* __restrict__ P,
-                  const MlpDev* __restrict__ N, const BatchGeo* __restrict__ geos, const uint8_t* __restrict__ active,
-                  const BatchChanGeo* __restrict__ cgeo, const uint8_t* __restrict__ states,
-                  const uint32_t* __restrict__ nb_start, const uint32_t* __restrict__ nb, uint32_t nchan,
-                  double* __restrict__ cpred, uint32_t* __restrict__ flags)
+                  const MlpDev* __restrict__ N, const MlpDev* __restrict__ F, const BatchGeo* __restrict__ geos,
+                  const uint8_t* __restrict__ active, const BatchChanGeo* __restrict__ cgeo,
+                  const uint8_t* __restrict__ states, const uint32_t* __restrict__ nb_start,
+                  const uint32_t* __restrict__ nb, uint32_t nchan, double* __restrict__ cpred,
+                  double* __restrict__ fpred, uint32_t* __restrict__ flags)
 {
   const int lane = threadIdx.x & 63, g = lane >> 5;
   const uint64_t tile = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
@@ -321,25 +329,32 @@ k_closure_predict(const T* __restrict__ u, const uint8_t* __restrict__ ids, cons
   const uint32_t cc = valid ? c : 0;
   const BatchChanGeo cg = cgeo[cc];
   const BatchGeo gg = geos[cg.inst];
-  bool oor = false;
-  bf16x8 x[2];
+  float raw[2][8];
 #pragma unroll
   for (int ch = 0; ch < 2; ch++)
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-      const int k = 16 * ch + 8 * g + j;
-      const float raw = valid ? closure_raw<T>(k, cc, cg, gg, u, ids, P, states, nb_start, nb) : 0.0f;
-      x[ch][j] = to_bf16(valid ? closure_input(N, k, raw, oor) : 0.0f);
-    }
-  f32x16 z1[2], z2[2];
-  bf16x8 h1b[2][2];
-  const float y = mlp_forward(N, [&](int seg) { return mlp_opnd(N, seg, lane); }, x, z1, z2, h1b,
-                              [](int, int, float) {});
+    for (int j = 0; j < 8; j++)
+      raw[ch][j] = valid ? closure_raw<T>(16 * ch + 8 * g + j, cc, cg, gg, u, ids, P, states, nb_start, nb) : 0.0f;
+  auto predict = [&](const MlpDev* __restrict__ M, bool& oor) {
+    bf16x8 x[2];
+#pragma unroll
+    for (int ch = 0; ch < 2; ch++)
+#pragma unroll
+      for (int j = 0; j < 8; j++)
+        x[ch][j] = to_bf16(valid ? closure_input(M, 16 * ch + 8 * g + j, raw[ch][j], oor) : 0.0f);
+    f32x16 z1[2], z2[2];
+    bf16x8 h1b[2][2];
+    return mlp_forward(M, [&](int seg) { return mlp_opnd(M, seg, lane); }, x, z1, z2, h1b, [](int, int, float) {});
+  };
+  bool oor = false, oor_f = false;
+  const float y = predict(N, oor);
+  const float yf = F ? predict(F, oor_f) : 0.0f;
   const bool oor_any = oor || __shfl_xor((int)oor, 32, 64);
   if (g != 0 || !valid) return;
   if (active && !active[cg.inst]) return;
-  if (flags && oor_any) atomicAdd(&flags[cg.inst], 1u);
+  if (oor_any) atomicAdd(&flags[cg.inst], 1u);
   cpred[c] = exp((double)y * N->out_std + N->out_mean);
+  if (F) fpred[c] = exp((double)yf * F->out_std + F->out_mean);
 }
 
 /* Output reductions of every realisation: one block per realisation, a
@@ -654,12 +669,33 @@ int upload_tiles(cs_batch* b)
   return CS_OK;
 }
 
+template <typename T, int NS, int PF, bool NT, uint32_t FM = BS_DYN>
+void launch_bstencil_k(cs_batch* b, const void* uin, void* uout, uint32_t dmask, uint32_t react)
+{
+  hipLaunchKernelGGL((k_batch_stencil<T, NS, PF, NT, FM>), dim3(b->grid), dim3(BX, b->launch.rows), 0, b->stream,
+                     (const T*)uin, (T*)uout, b->d_ids, (const DevParams<T>*)b->d_params, b->d_geo, b->d_tiles,
+                     b->ntiles, dmask, react, b->d_queue);
+}
+
+/* The step masks of the reference's five-slot species layout (Ca, dye,
+ * CMDN-CSQN, ATP, TRPN: Ca and ATP every step, the dye and CMDN on their
+ * own multiples) run kernels compiled for them in FP32 with one-plane
+ * prefetch; other masks, layouts and FP64 read the mask at run time. */
 template <typename T, int NS, int PF, bool NT>
 void launch_bstencil_v(cs_batch* b, const void* uin, void* uout, uint32_t dmask, uint32_t react)
 {
-  hipLaunchKernelGGL((k_batch_stencil<T, NS, PF, NT>), dim3(b->grid), dim3(BX, b->launch.rows), 0, b->stream,
-                     (const T*)uin, (T*)uout, b->d_ids, (const DevParams<T>*)b->d_params, b->d_geo, b->d_tiles,
-                     b->ntiles, dmask, react, b->d_queue);
+  if constexpr (sizeof(T) == 4 && NS == 5 && PF == 1) {
+    if (react == 2) {
+      switch (dmask) {
+        case 0x9: return launch_bstencil_k<T, NS, PF, NT, 0x9>(b, uin, uout, dmask, react);
+        case 0xB: return launch_bstencil_k<T, NS, PF, NT, 0xB>(b, uin, uout, dmask, react);
+        case 0xD: return launch_bstencil_k<T, NS, PF, NT, 0xD>(b, uin, uout, dmask, react);
+        case 0xF: return launch_bstencil_k<T, NS, PF, NT, 0xF>(b, uin, uout, dmask, react);
+        default: break;
+      }
+    }
+  }
+  launch_bstencil_k<T, NS, PF, NT>(b, uin, uout, dmask, react);
 }
 
 template <typename T, int NS>
@@ -732,12 +768,8 @@ void launch_predict(cs_batch* b, const void* u)
   const unsigned blocks = (b->nchan_total + 127) / 128;
   if (!blocks) return;
   hipLaunchKernelGGL(k_closure_predict<T>, dim3(blocks), dim3(256), 0, b->stream, (const T*)u, b->d_ids,
-                     (const DevParams<T>*)b->d_params, b->d_net, b->d_geo, b->d_active, b->d_cgeo, b->d_states,
-                     b->d_nb_start, b->d_nb, b->nchan_total, b->d_cpred, b->d_flags);
-  if (b->d_fnet)
-    hipLaunchKernelGGL(k_closure_predict<T>, dim3(blocks), dim3(256), 0, b->stream, (const T*)u, b->d_ids,
-                       (const DevParams<T>*)b->d_params, b->d_fnet, b->d_geo, b->d_active, b->d_cgeo, b->d_states,
-                       b->d_nb_start, b->d_nb, b->nchan_total, b->d_fpred, (uint32_t*)nullptr);
+                     (const DevParams<T>*)b->d_params, b->d_net, b->d_fnet, b->d_geo, b->d_active, b->d_cgeo,
+                     b->d_states, b->d_nb_start, b->d_nb, b->nchan_total, b->d_cpred, b->d_fpred, b->d_flags);
 }
 
 /* Steps [first, first + n) on the stream: gating of the first step, then
@@ -1484,6 +1516,14 @@ int cs_batch_set_launch(cs_batch* b, const cs_batch_launch* in)
   return cs_batch_set_active(b, act.data());
 }
 
+int cs_batch_step_period(const cs_batch* b, uint64_t* period)
+{
+  if (!b || !period) return set_err(CS_EINVAL, "NULL argument");
+  uint64_t P = b_period(b);
+  *period = P % 2 ? 2 * P : P;
+  return CS_OK;
+}
+
 int cs_batch_set_graph(cs_batch* b, uint32_t on)
 {
   if (!b) return set_err(CS_EINVAL, "ensemble is NULL");
@@ -1509,7 +1549,8 @@ int cs_batch_time_steps(cs_batch* b, uint64_t first, uint64_t n, double* total_m
   int rc;
   if (b->use_graph) {
     /* the whole range as the caller runs it (graphs included); the
-     * stencil's share comes from a second, event-bracketed range */
+     * stencil's share is not separated (reported as 0): time a range with
+     * graphs off for it */
     rc = b->esize == 8 ? b_step<double>(b, first, n, nullptr, 0) : b_step<float>(b, first, n, nullptr, 0);
   } else {
     rc = b->esize == 8 ? b_run<double>(b, first, n, nullptr, ev.data()) : b_run<float>(b, first, n, nullptr, ev.data());

@@ -40,7 +40,8 @@ constexpr int BX = 64;          /* one wavefront along x                  */
 constexpr int GATE_THREADS = 256;
 constexpr int FLUX_THREADS = 256;
 constexpr int BRIDGE_THREADS = 1024;
-constexpr size_t F2_MAX_ROWS = 16384;   /* rows and channels k_bridge's one block takes */
+constexpr size_t F2_ONE_BLOCK = 16384;  /* rows and channels k_bridge runs in one block; more
+                                          run each phase as a launch of many blocks */
 constexpr int STAT_THREADS = 256;
 constexpr int STAT_BLOCKS = 4096;
 

@@ -961,15 +961,22 @@ k_step2(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__
  *     level 1.
  * Either part may be off.  The gating (gate_one) and flux (flux_j) code,
  * the entry order and the skip rules are k_gate's and k_flux's, so the
- * results are the same. */
+ * results are the same.  phases selects the phases this launch runs
+ * (BR_TAIL_GATE, BR_TAIL_FLUX, BR_HEAD_GATE, BR_HEAD_ROWS): one block runs
+ * them all with barriers between them; a problem with more channels or
+ * rows than a block takes well runs each phase as its own launch over
+ * many blocks, the launches on one stream keeping the order. */
+constexpr uint32_t BR_TAIL_GATE = 1, BR_TAIL_FLUX = 2, BR_HEAD_GATE = 4, BR_HEAD_ROWS = 8, BR_ALL = 15;
+
 template <typename T>
 __global__ void __launch_bounds__(BRIDGE_THREADS)
 k_bridge(T* __restrict__ u, const DevParams<T>* __restrict__ P, const BChan* __restrict__ chans, uint32_t nchan,
          const BRow* __restrict__ rows, uint32_t nrows, const BEnt* __restrict__ ents, const T* __restrict__ l1,
          uint8_t* __restrict__ states, uint8_t* __restrict__ recruited, double* __restrict__ fsum,
          uint8_t* __restrict__ fany, F2Step tail, uint32_t do_tail, F2Step head, uint32_t do_head, uint32_t D,
-         uint32_t seed, uint32_t instance, const double* __restrict__ uniforms)
+         uint32_t seed, uint32_t instance, const double* __restrict__ uniforms, uint32_t phases)
 {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nth = gridDim.x * blockDim.x;
   __shared__ FluxParams F[MAXF];
   {
     const uint32_t words = (uint32_t)(sizeof(FluxParams) * MAXF / sizeof(uint64_t));
@@ -982,7 +989,7 @@ k_bridge(T* __restrict__ u, const DevParams<T>* __restrict__ P, const BChan* __r
 
   auto gate = [&](const F2Step& st, auto ca) {
     const double t = (double)st.ti * dt;
-    for (uint32_t c = threadIdx.x; c < nchan; c += blockDim.x) {
+    for (uint32_t c = tid; c < nchan; c += nth) {
       const BChan b = chans[c];
       const FluxParams& Fp = F[b.flux];
       const uint32_t k = Fp.state_off + b.ch;
@@ -1012,10 +1019,10 @@ k_bridge(T* __restrict__ u, const DevParams<T>* __restrict__ P, const BChan* __r
   };
 
   if (do_tail) {
-    if (tail.gate) gate(tail, [&](const BChan& b) { return (double)l1[b.slot_to]; });
+    if (tail.gate && (phases & BR_TAIL_GATE)) gate(tail, [&](const BChan& b) { return (double)l1[b.slot_to]; });
     __syncthreads();
-    if (tail.fluxmask)
-      for (uint32_t r = threadIdx.x; r < nrows; r += blockDim.x) {
+    if (tail.fluxmask && (phases & BR_TAIL_FLUX))
+      for (uint32_t r = tid; r < nrows; r += nth) {
         double sum;
         if (row_flux(r, tail.fluxmask,
                      [&](const BEnt& x, bool to) { return (double)l1[to ? x.slot_to : x.slot_from]; }, sum)) {
@@ -1026,15 +1033,17 @@ k_bridge(T* __restrict__ u, const DevParams<T>* __restrict__ P, const BChan* __r
     __syncthreads();
   }
   if (do_head) {
-    if (head.gate) gate(head, [&](const BChan& b) { return (double)u[b.elem_to]; });
+    if (head.gate && (phases & BR_HEAD_GATE)) gate(head, [&](const BChan& b) { return (double)u[b.elem_to]; });
     __syncthreads();
-    for (uint32_t r = threadIdx.x; r < nrows; r += blockDim.x) {
-      double sum;
-      const bool any = row_flux(r, head.fluxmask,
-                                [&](const BEnt& x, bool to) { return (double)u[to ? x.elem_to : x.elem_from]; }, sum);
-      fsum[r] = sum;
-      fany[r] = any ? 1 : 0;
-    }
+    if (phases & BR_HEAD_ROWS)
+      for (uint32_t r = tid; r < nrows; r += nth) {
+        double sum;
+        const bool any = row_flux(r, head.fluxmask,
+                                  [&](const BEnt& x, bool to) { return (double)u[to ? x.elem_to : x.elem_from]; },
+                                  sum);
+        fsum[r] = sum;
+        fany[r] = any ? 1 : 0;
+      }
   }
 }
 
@@ -1796,11 +1805,29 @@ template <typename T>
 void launch_bridge(cs_context* c, void* u, const F2Step& tail, bool do_tail, const F2Step& head, bool do_head,
                    const double* d_uni)
 {
-  hipLaunchKernelGGL(k_bridge<T>, dim3(1), dim3(BRIDGE_THREADS), 0, c->stream, (T*)u,
-                     (const DevParams<T>*)c->d_params, (const BChan*)c->d_bchan, c->nbchan,
-                     (const BRow*)c->d_brow, c->nrows, (const BEnt*)c->d_bent, (const T*)c->d_f2_l1, c->d_states,
-                     c->d_recruited, c->d_f2_fsum, c->d_f2_fany, tail, do_tail ? 1u : 0u, head, do_head ? 1u : 0u,
-                     (uint32_t)c->prob.stochastic_substep, (uint32_t)c->prob.seed, (uint32_t)c->prob.instance, d_uni);
+  auto go = [&](uint32_t blocks, uint32_t phases) {
+    hipLaunchKernelGGL(k_bridge<T>, dim3(blocks), dim3(BRIDGE_THREADS), 0, c->stream, (T*)u,
+                       (const DevParams<T>*)c->d_params, (const BChan*)c->d_bchan, c->nbchan,
+                       (const BRow*)c->d_brow, c->nrows, (const BEnt*)c->d_bent, (const T*)c->d_f2_l1, c->d_states,
+                       c->d_recruited, c->d_f2_fsum, c->d_f2_fany, tail, do_tail ? 1u : 0u, head,
+                       do_head ? 1u : 0u, (uint32_t)c->prob.stochastic_substep, (uint32_t)c->prob.seed,
+                       (uint32_t)c->prob.instance, d_uni, phases);
+  };
+  const size_t most = std::max<size_t>(c->nrows, c->nbchan);
+  if (most <= F2_ONE_BLOCK) {
+    go(1, BR_ALL);
+    return;
+  }
+  /* each phase over enough blocks for its channels or rows */
+  auto blocks = [](size_t n) { return (uint32_t)std::max<size_t>(1, (n + BRIDGE_THREADS - 1) / BRIDGE_THREADS); };
+  if (do_tail) {
+    if (tail.gate) go(blocks(c->nbchan), BR_TAIL_GATE);
+    if (tail.fluxmask) go(blocks(c->nrows), BR_TAIL_FLUX);
+  }
+  if (do_head) {
+    if (head.gate) go(blocks(c->nbchan), BR_HEAD_GATE);
+    go(blocks(c->nrows), BR_HEAD_ROWS);
+  }
 }
 
 uint64_t gates_in(const cs_context* c, uint64_t first, uint64_t n)
@@ -1999,8 +2026,6 @@ int build_flux_rows(cs_context* c, const cs_problem* p)
       c->fsm |= 1u << p->fluxes[f].species;
       if (p->fluxes[f].species != 0) why("a flux acts on a species other than 0");
     }
-  /* one block gates every channel and sums every row between passes */
-  if (c->nrows > F2_MAX_ROWS || c->init_states.size() > F2_MAX_ROWS) why("more than 16384 flux rows or channels");
   std::vector<uint64_t> keys;
   keys.reserve(pairs_dev.size());
   for (uint32_t v : pairs_dev) keys.push_back((uint64_t)(v % g.plane) << 32 | (uint64_t)(v / g.plane));

@@ -198,11 +198,12 @@ int cs_time_passes(cs_context* ctx, uint64_t first_time_ind, uint64_t nsteps, do
 
 /* 1 when the problem admits fused two-step passes, else 0 with the
  * reason in why (if given).  They need at most 7 species, every boundary
- * flux acting on species 0, at most 16384 flux rows (species, voxel) and
- * at most 16384 channels (one k_bridge block gates and sums them all),
- * and a z-group of every species of a plane small enough that the
- * pass's 32-bit buffer offsets cover at least one plane plus its four
- * neighbour planes.  No reference counterpart. */
+ * flux acting on species 0, and a z-group of every species of a plane
+ * small enough that the pass's 32-bit buffer offsets cover at least one
+ * plane plus its four neighbour planes.  Up to 16384 flux rows (species,
+ * voxel) and channels one k_bridge block gates and sums them between
+ * passes; beyond that each of its phases is a launch of many blocks.
+ * No reference counterpart. */
 int cs_fused_available(const cs_context* ctx, char* why, size_t why_len);
 
 /* Launch shape of the fused stencil/reaction kernel: blocks of 64 x rows
@@ -375,6 +376,10 @@ int cs_batch_autotune(cs_batch* b, uint64_t first_time_ind, uint32_t reps, cs_ba
  * least common multiple of every substep, doubled if odd) as captured HIP
  * graphs, with Philox gating.  No reference counterpart. */
 int cs_batch_set_graph(cs_batch* b, uint32_t on);
+/* Steps of one captured period (the least common multiple of every
+ * substep, doubled if odd): step counts that are multiples of it replay
+ * graphs only. */
+int cs_batch_step_period(const cs_batch* b, uint64_t* period);
 /* Time nsteps steps: total ms, and the mean stencil pass (0 with graphs). */
 int cs_batch_time_steps(cs_batch* b, uint64_t first_time_ind, uint64_t nsteps, double* total_ms,
                         double* stencil_ms);

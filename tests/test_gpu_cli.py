@@ -21,18 +21,28 @@ def _say(msg):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("h,steps", [(12.0, 200), (6.0, 30)])
-def test_g1_reference_geometry_fp64(h, steps):
+@pytest.mark.parametrize("h,steps,fuse", [(12.0, 200, 0), (12.0, 200, 1), (6.0, 30, 0), (6.0, 31, 1)])
+def test_g1_reference_geometry_fp64(h, steps, fuse):
     """The shipped G1 geometry (84^3 at 12 nm, 31416 SERCA pairs, 50 RyRs)
     with the shipped sr1100/ryr55 species file, RyR 24 forced open and
-    closed at 0.5 ms: GPU vs oracle to 1e-10 per voxel."""
+    closed at 0.5 ms: GPU vs oracle to 1e-10 per voxel, in one-step and in
+    fused passes (more flux rows than one bridge block takes, so every
+    bridge phase runs as a launch of many blocks)."""
     g = load_geometry_fixture("g1_nonP_0")
     m = load_model_fixture("params_sr1100_ryr55")
     opts = cs.Options(h=h, seed=11, open={"ryr": [24]}, t_close={"ryr": 0.5})
-    _say(f"case: G1 h={h} steps={steps}")
+    _say(f"case: G1 h={h} steps={steps} fuse={fuse}")
     orc = Oracle(g, m, opts)
     sim = cs.Simulation(g, m, opts)
+    if fuse:
+        ok, why = sim.fused_available()
+        assert ok, why
+    sim.set_launch_config(fuse=fuse)
     sim.step(steps)
+    if fuse:
+        _, _, passes, fused = sim.time_passes(2)
+        orc.step(2)
+        assert fused == passes == 1
     orc.step(steps)
     for s in range(orc.ns):
         ok, err = species_close(sim.species(s), orc.species(s), 1e-10)

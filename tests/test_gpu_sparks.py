@@ -1,8 +1,10 @@
 """Spark statistics of GPU realisations against the oracle's.
 
-The oracle realisations (tests/golden/spark_stats_fast_dyad.json) gate
-with the reference's Mersenne Twister order in FP64; the GPU ones use the
-Philox stream with FP32 fields.  The random streams differ, so the
+The oracle realisations (tests/golden/spark_stats_fast_dyad.json: a
+fast-gating dyad, 1000 seeds; tests/golden/spark_stats_lattice_dyad.json:
+the lattice dyad with the shipped kinetics, 200 seeds) gate with the
+reference's Mersenne Twister order in FP64; the GPU ones use the Philox
+stream with FP32 fields.  The random streams differ, so the
 realisations differ one by one; the distributions of peak dF/F0, duration
 and RyRs recruited must agree (two-sample Kolmogorov-Smirnov, p > 0.05)."""
 import json
@@ -20,19 +22,13 @@ from conftest import REPO
 
 pytestmark = pytest.mark.gpu
 FIXTURE = os.path.join(REPO, "tests", "golden", "spark_stats_fast_dyad.json")
+LATTICE = os.path.join(REPO, "tests", "golden", "spark_stats_lattice_dyad.json")
 
 
-def test_spark_statistics_match_the_oracle_distributions():
-    """All realisations run as one ensemble (one member per seed).  Three
-    statistics are tested at once, so each must pass at 0.05 / 3
-    (Bonferroni), which keeps the family's false-alarm rate under 5%."""
-    with open(FIXTURE) as f:
-        fx = json.load(f)
+def _compare(fx, d, label):
     ref = fx["realisations"]
-    d = dyad.fast_spark_dyad()
     n = len(ref)
-    assert n >= 1000
-    print(f"spark realisations: {n} GPU members (Philox, FP32) against {n} oracle seeds (MT, FP64)", flush=True)
+    print(f"{label}: {n} GPU members (Philox, FP32) against {n} oracle seeds (MT, FP64)", flush=True)
     ens = cs.Ensemble([(d.geometry, d.model, cs.Options(seed=s, precision="f32", open={"ryr": fx["open"]["ryr"]}))
                        for s in range(1, n + 1)], instances=list(range(n)))
     got = sparks.realise_ensemble(ens, fx["nsteps"], fx["save_every"], fx["species"], fx["offsets"], d.model.conv)
@@ -48,3 +44,25 @@ def test_spark_statistics_match_the_oracle_distributions():
         if not res.pvalue > alpha:
             failed.append(f"{key}: KS p = {res.pvalue:.4f}")
     assert not failed, "; ".join(failed)
+
+
+def test_spark_statistics_match_the_oracle_distributions():
+    """All realisations run as one ensemble (one member per seed).  Three
+    statistics are tested at once, so each must pass at 0.05 / 3
+    (Bonferroni), which keeps the family's false-alarm rate under 5%."""
+    with open(FIXTURE) as f:
+        fx = json.load(f)
+    assert len(fx["realisations"]) >= 1000
+    _compare(fx, dyad.fast_spark_dyad(), "fast-gating dyad")
+
+
+@pytest.mark.skipif(not os.path.exists(LATTICE), reason="spark_stats_lattice_dyad.json not generated "
+                    "(tests/golden/make_spark_fixture.py lattice)")
+def test_lattice_spark_statistics_with_the_shipped_kinetics():
+    """The 64x64x25 lattice dyad with the shipped RyR kinetics (Hill
+    opening at K 70 / 35 uM up to 0.3/ms, closing 0.5/ms), the first 2 ms
+    of 200 sparks (tests/golden/spark_stats_lattice_dyad.json)."""
+    with open(LATTICE) as f:
+        fx = json.load(f)
+    assert len(fx["realisations"]) >= 200
+    _compare(fx, dyad.lattice_dyad(), "lattice dyad, shipped kinetics")
