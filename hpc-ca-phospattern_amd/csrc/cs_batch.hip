@@ -329,12 +329,42 @@ k_closure_predict(const T* __restrict__ u, const uint8_t* __restrict__ ids, cons
   const uint32_t cc = valid ? c : 0;
   const BatchChanGeo cg = cgeo[cc];
   const BatchGeo gg = geos[cg.inst];
+  /* the open-channel counts (inputs 28, 29, in the upper lane group):
+   * one pass over the neighbour list for both, four entries at a time so
+   * that four state loads are in flight */
+  uint32_t n1 = 0, n2 = 0;
+  if (valid && g == 1) {
+    const uint32_t e1 = nb_start[cc + 1];
+    uint32_t e = nb_start[cc];
+    for (; e + 4 <= e1; e += 4) {
+      uint32_t v[4], st[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) v[q] = nb[e + q];
+#pragma unroll
+      for (int q = 0; q < 4; q++) st[q] = states[v[q] & ~NB_FAR];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        n2 += st[q];
+        n1 += (v[q] & NB_FAR) ? 0u : st[q];
+      }
+    }
+    for (; e < e1; e++) {
+      const uint32_t v = nb[e], st = states[v & ~NB_FAR];
+      n2 += st;
+      n1 += (v & NB_FAR) ? 0u : st;
+    }
+  }
   float raw[2][8];
 #pragma unroll
   for (int ch = 0; ch < 2; ch++)
 #pragma unroll
-    for (int j = 0; j < 8; j++)
-      raw[ch][j] = valid ? closure_raw<T>(16 * ch + 8 * g + j, cc, cg, gg, u, ids, P, states, nb_start, nb) : 0.0f;
+    for (int j = 0; j < 8; j++) {
+      const int k = 16 * ch + 8 * g + j;
+      raw[ch][j] = !valid ? 0.0f
+                 : k == 28 ? (float)n1
+                 : k == 29 ? (float)n2
+                 : closure_raw<T>(k, cc, cg, gg, u, ids, P, states, nb_start, nb);
+    }
   auto predict = [&](const MlpDev* __restrict__ M, bool& oor) {
     bf16x8 x[2];
 #pragma unroll
