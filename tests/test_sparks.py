@@ -6,6 +6,7 @@ import os
 import numpy as np
 import pytest
 
+import csparks as cs
 import dyad
 import sparks
 from conftest import REPO
@@ -53,3 +54,25 @@ def test_fixture_holds_the_oracle_realisations():
     rec = np.array([r["recruited"] for r in rows])
     # bimodal by construction: the first channel alone, or the whole cluster
     assert 0.2 < np.mean(rec == 1) < 0.5 and 0.5 < np.mean(rec == 16) < 0.8
+
+
+LATTICE = os.path.join(REPO, "tests", "golden", "spark_stats_lattice_dyad.json")
+
+
+@pytest.mark.skipif(not os.path.exists(LATTICE), reason="spark_stats_lattice_dyad.json not generated")
+def test_lattice_fixture_holds_the_shipped_kinetics_runs():
+    """200 oracle realisations of the lattice dyad with the shipped kinetics
+    over the first 2 ms: every run starts with the forced-open channel, so
+    at least one channel is recruited, and the step plan is the lattice
+    dyad's (dt 0.95 h^2 / (6 sigma_max) at 6 nm)."""
+    with open(LATTICE) as f:
+        d = json.load(f)
+    rows = d["realisations"]
+    assert len(rows) >= 200 and d["rng"] == "mt" and d["model"] == "dyad.lattice_dyad()"
+    assert [r["seed"] for r in rows] == list(range(1, len(rows) + 1))
+    dt = 0.95 * 36.0 / (6 * 220000.0)
+    assert d["nsteps"] == cs.host_lib().cs_count_steps(d["tstop_ms"], dt)
+    assert d["save_every"] == cs.host_lib().cs_save_interval(d["save_ms"], dt)
+    rec = np.array([r["recruited"] for r in rows])
+    assert rec.min() >= 1
+    assert np.all(np.isfinite([r["peak"] for r in rows])) and np.all(np.array([r["duration"] for r in rows]) >= 0)
