@@ -140,13 +140,17 @@ def release_ratio(fine_from, fine_to, coarse_from, coarse_to, k_fine, k_coarse, 
 
 def paired_samples(dyads: Sequence, seeds: Sequence[int], fine_h: float = 3.0, coarse_h: float = 12.0,
                    coarse_steps: int = 400, sample_every: int = 2, warmup: int = 10,
-                   kinetics: Tuple[float, float] = (20.0, 40.0), precision: str = "f32") -> PairedSamples:
+                   kinetics: Tuple[float, float] = (20.0, 40.0), precision: str = "f32",
+                   say=None) -> PairedSamples:
     """Run the dyads (geometries at coarse_h) at fine_h and coarse_h side by
-    side with one channel schedule and collect (features, labels)."""
+    side with one channel schedule and collect (features, labels); say,
+    if given, receives a progress line every 50 coarse steps."""
     members = []
     for d, s in zip(dyads, seeds):
         m = schedule_model(d.model, *kinetics)
         members.append((d.geometry, m, s, d.open_index))
+    if say is not None:
+        say(f"building the {fine_h:g} nm and {coarse_h:g} nm ensembles of {len(members)} dyads")
     fine = cs.Ensemble([(g, m, cs.Options(h=fine_h, seed=s, open={"ryr": [k]}, precision=precision))
                         for g, m, s, k in members], instances=list(range(len(members))))
     coarse = cs.Ensemble([(g, m, cs.Options(h=coarse_h, seed=s, open={"ryr": [k]}, precision=precision))
@@ -160,10 +164,15 @@ def paired_samples(dyads: Sequence, seeds: Sequence[int], fine_h: float = 3.0, c
     owner = coarse.channel_owner()
     kf, kc = release_factors(fine, fine_h), release_factors(coarse, coarse_h)
     raw, yf, yc, own, op, rr = [], [], [], [], [], []
+    if say is not None:
+        _, nvf, _ = fine.info()
+        say(f"paired ensembles: {len(members)} dyads, {nvf} fine voxels, {steps_per} fine steps per coarse step")
     for k in range(1, coarse_steps + 1):
         fine.step(steps_per)
         coarse.copy_channel_states(fine)
         coarse.step(1)
+        if say is not None and k % 50 == 0:
+            say(f"  coarse step {k}/{coarse_steps}")
         if k >= warmup and (k - warmup) % sample_every == 0:
             raw.append(coarse.closure_features())
             yf.append(fine.channel_ca())
