@@ -76,3 +76,53 @@ def test_lattice_fixture_holds_the_shipped_kinetics_runs():
     rec = np.array([r["recruited"] for r in rows])
     assert rec.min() >= 1
     assert np.all(np.isfinite([r["peak"] for r in rows])) and np.all(np.array([r["duration"] for r in rows]) >= 0)
+
+
+@pytest.mark.skipif(not os.path.exists(LATTICE), reason="spark_stats_lattice_dyad.json not generated")
+def test_lattice_atom_is_the_channel_that_never_closes():
+    """The lattice dyad's RyR closes at a constant 0.5/ms, so the forced-open
+    channel stays open through the 2 ms window with probability exp(-1);
+    with no other channel recruited that run is one deterministic
+    trajectory, repeated exactly by every such seed.  Its share of the 200
+    oracle runs must match exp(-1) within 4 binomial standard deviations."""
+    with open(LATTICE) as f:
+        d = json.load(f)
+    rows = d["realisations"]
+    peak = np.array([r["peak"] for r in rows])
+    vals, counts = np.unique(peak, return_counts=True)
+    assert counts.max() > 1
+    atom = vals[np.argmax(counts)]
+    share = float(np.mean(peak == atom))
+    p = math.exp(-0.5 * d["tstop_ms"])
+    assert abs(share - p) < 4 * math.sqrt(p * (1 - p) / len(rows)), (share, p)
+    # the same runs carry the same duration and recruit only the trigger
+    dur = np.array([r["duration"] for r in rows])[peak == atom]
+    rec = np.array([r["recruited"] for r in rows])[peak == atom]
+    assert np.all(dur == dur[0]) and np.all(rec == 1)
+    # among the runs that recruit nobody the atom's peak is the largest:
+    # its channel never closed
+    alone = np.array([r["recruited"] for r in rows]) == 1
+    assert atom >= np.max(peak[alone & (peak != atom)])
+
+
+@pytest.mark.skipif(not os.path.exists(LATTICE), reason="spark_stats_lattice_dyad.json not generated")
+def test_ks_comparison_snaps_fp32_copies_of_an_atom():
+    """Moving the oracle's atom by 3e-7 (an FP32 rounding of the same
+    trajectory) makes the plain two-sample KS test report the atom's mass
+    as D and fail; after snapping onto the oracle's atoms the same sample
+    passes, while a real shift of the continuous part still fails."""
+    from scipy.stats import ks_2samp
+    from helpers import snap_to_atoms
+    with open(LATTICE) as f:
+        rows = json.load(f)["realisations"]
+    ref = np.array([r["peak"] for r in rows])
+    vals, counts = np.unique(ref, return_counts=True)
+    atom = vals[np.argmax(counts)]
+    got = ref.copy()
+    got[got == atom] *= 1 + 3e-7
+    assert ks_2samp(got, ref).pvalue < 1e-6
+    snapped, worst, moved, atoms = snap_to_atoms(got, ref)
+    assert moved == int(np.sum(ref == atom)) and atom in atoms and worst < 1e-6
+    assert ks_2samp(snapped, ref).pvalue > 0.5
+    shifted = np.where(ref == atom, ref, ref * 0.8)
+    assert ks_2samp(snap_to_atoms(shifted, ref)[0], ref).pvalue < 0.05 / 3
