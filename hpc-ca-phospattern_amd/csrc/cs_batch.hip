@@ -259,13 +259,20 @@ __device__ inline float closure_raw(int k, uint32_t c, const BatchChanGeo& cg, c
   return 0.0f;
 }
 
-/* Normalised input: logarithm for the concentrations, then the training
- * set's standardisation; flags an input outside the monitored range. */
-__device__ inline float closure_input(const MlpDev* __restrict__ N, int k, float raw, bool& out_of_range)
+/* Feature k of a raw input: the logarithm of a concentration (k < 28),
+ * a count as it is (28, 29). */
+__device__ inline float closure_feature(int k, float raw)
+{
+  return k < 28 ? logf(fmaxf(raw, 1e-6f)) : raw;
+}
+
+/* Normalised input of feature f: the training set's standardisation (by
+ * the reciprocal of the deviation, within an FP32 rounding of the
+ * quotient); flags an input outside the monitored range. */
+__device__ inline float closure_input(const MlpDev* __restrict__ N, int k, float f, bool& out_of_range)
 {
   if (k >= MLP_IN) return 0.0f;
-  const float f = k < 28 ? logf(fmaxf(raw, 1e-6f)) : raw;
-  const float x = (f - N->in_mean[k]) / N->in_std[k];
+  const float x = (f - N->in_mean[k]) * __builtin_amdgcn_rcpf(N->in_std[k]);
   if (x < N->in_lo[k] - N->margin || x > N->in_hi[k] + N->margin) out_of_range = true;
   return x;
 }
@@ -354,16 +361,17 @@ k_closure_predict(const T* __restrict__ u, const uint8_t* __restrict__ ids, cons
       n1 += (v & NB_FAR) ? 0u : st;
     }
   }
-  float raw[2][8];
+  float feat[2][8];
 #pragma unroll
   for (int ch = 0; ch < 2; ch++)
 #pragma unroll
     for (int j = 0; j < 8; j++) {
       const int k = 16 * ch + 8 * g + j;
-      raw[ch][j] = !valid ? 0.0f
+      feat[ch][j] = !valid ? 0.0f
                  : k == 28 ? (float)n1
                  : k == 29 ? (float)n2
                  : closure_raw<T>(k, cc, cg, gg, u, ids, P, states, nb_start, nb);
+      feat[ch][j] = closure_feature(k, feat[ch][j]);   /* once, for both networks */
     }
   auto predict = [&](const MlpDev* __restrict__ M, bool& oor) {
     bf16x8 x[2];
@@ -371,7 +379,7 @@ k_closure_predict(const T* __restrict__ u, const uint8_t* __restrict__ ids, cons
     for (int ch = 0; ch < 2; ch++)
 #pragma unroll
       for (int j = 0; j < 8; j++)
-        x[ch][j] = to_bf16(valid ? closure_input(M, 16 * ch + 8 * g + j, raw[ch][j], oor) : 0.0f);
+        x[ch][j] = to_bf16(valid ? closure_input(M, 16 * ch + 8 * g + j, feat[ch][j], oor) : 0.0f);
     f32x16 z1[2], z2[2];
     bf16x8 h1b[2][2];
     return mlp_forward(M, [&](int seg) { return mlp_opnd(M, seg, lane); }, x, z1, z2, h1b, [](int, int, float) {});

@@ -77,3 +77,31 @@ def test_bf16_forward_stays_near_fp64(seed):
     y64, _ = mo.forward(p, x)
     y16, _ = mo.forward(p, x, bf16_ops=True)
     assert np.max(np.abs(y16 - y64)) < 0.05 * max(1.0, np.max(np.abs(y64)))
+
+
+def test_device_erf_constants_meet_their_published_bound():
+    """The device GELU takes erf from Abramowitz and Stegun 7.1.26
+    (|error| <= 1.5e-7).  The constants are read from the device source
+    (cs_mlp.h, mlp_erf) and evaluated in the same order in FP64, so a
+    mistyped digit or a term out of order fails against scipy's erf."""
+    import os
+    import re
+    from scipy.special import erf
+    from conftest import REPO
+    src = open(os.path.join(REPO, "hpc-ca-phospattern_amd", "csrc", "cs_mlp.h")).read()
+    body = src[src.index("__device__ inline float mlp_erf"):]
+    body = body[:body.index("\n}\n")]
+    num = r"(-?\d+\.\d+)f"
+    p = float(re.search(r"fmaf\(" + num + r", ax, 1\.0f\)", body).group(1))
+    horner = [float(re.search(r"float p = " + num, body).group(1))]
+    horner += [float(v) for v in re.findall(r"p = fmaf\(p, t, " + num + r"\)", body)]
+    assert len(horner) == 5
+    x = np.linspace(-6.0, 6.0, 200001)
+    ax = np.abs(x)
+    t = 1.0 / (1.0 + p * ax)
+    poly = horner[0]
+    for a in horner[1:]:
+        poly = poly * t + a
+    approx = np.copysign(1.0 - poly * t * np.exp(-ax * ax), x)
+    err = float(np.max(np.abs(approx - erf(x))))
+    assert err <= 1.5e-7, err
