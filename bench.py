@@ -170,7 +170,8 @@ def ensemble_record(count=512, steps=100):
     return {"workload": f"{count} dyads, edges 256-512 nm at 6 nm, 9-100 RyRs, 0-50% pRyR, Philox, FP32, 20 ms runs",
             "realisations": count, "voxels": nv, "ms_per_step": round(ms, 4), "steps_per_realisation": run_steps,
             "realisations_per_s": round(rate, 4), "launch": launch, "graphs": True,
-            "stencil": {"kernel": "k_batch_stencil", "kernel_ms": round(stencil_ms, 5),
+            "stencil": {"kernel": "k_batch_stencil" if launch.get("persistent") else "k_batch_tiles",
+                        "kernel_ms": round(stencil_ms, 5),
                         "achieved": round(gbs, 1), "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4)}}
 
 

@@ -182,7 +182,7 @@ class CLaunchConfig(C.Structure):
 
 class CBatchLaunch(C.Structure):
     _fields_ = [("rows", C.c_uint32), ("zchunk", C.c_uint32), ("prefetch", C.c_uint32),
-                ("nontemporal", C.c_uint32)]
+                ("nontemporal", C.c_uint32), ("persistent", C.c_uint32)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

@@ -13,6 +13,9 @@
  *                    stencil, reactions, Dirichlet faces, fixed species),
  *                    tiles taken from eight per-XCD work queues so that
  *                    neighbouring tiles of a realisation share an L2;
+ *   k_batch_tiles    the same update with one block per tile, the tiles
+ *                    numbered in groups per XCD (the hardware schedules
+ *                    the blocks; no claims, no queues);
  *   k_batch_bridge   one wavefront per realisation: the boundary fluxes of
  *                    step t added to the new state (k_flux's rows and
  *                    order) and then the gating of step t+1 from that
@@ -161,6 +164,36 @@ k_batch_stencil(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __re
     if (atomicAdd(&queue[NQUEUE], 1u) == gridDim.x - 1) {
       for (int q = 0; q <= NQUEUE; q++) atomicExch(&queue[q], 0u);
     }
+  }
+}
+
+/* The stencil pass with one block per tile: block b takes tile
+ * xcd_tile(b) (groups of xcd consecutive tiles per XCD), so a tile costs
+ * no queue claim and no barrier before its first loads, and the hardware
+ * overlaps one block's start with the others' planes.  Each column's
+ * update is k_batch_stencil's, bit for bit. */
+template <typename T, int NS, int PF, bool NT, uint32_t FM = BS_DYN>
+__global__ void __launch_bounds__(BX * 8)
+k_batch_tiles(const T* __restrict__ u, T* __restrict__ un, const uint8_t* __restrict__ ids,
+              const DevParams<T>* __restrict__ P, const BatchGeo* __restrict__ geos,
+              const BatchTile* __restrict__ tiles, uint32_t ntiles, uint32_t dmask_rt, uint32_t react_rt, int xcd)
+{
+  const uint32_t dmask = FM != BS_DYN ? FM : dmask_rt, react = FM != BS_DYN ? 2u : react_rt;
+  const int t = xcd_tile((int)blockIdx.x, (int)ntiles, xcd);
+  if (t >= (int)ntiles) return;   /* whole padding blocks only: no barrier is skipped */
+  __shared__ BlockTables<T, NS> S;
+  const int nthr = BX * blockDim.y;
+  const int tid = threadIdx.y * BX + threadIdx.x;
+  const BatchTile tl = tiles[t];
+  const BatchGeo gg = geos[tl.inst];
+  S.load(P, tid, nthr);
+  __syncthreads();
+  const Geo g{gg.nx, gg.ny, gg.nz, gg.ns, gg.plane, gg.nv};
+  const uint32_t c = tl.col0 + (uint32_t)tid;
+  if (c < gg.plane) {
+    const int y = (int)(c / (uint32_t)g.nx), x = (int)(c - (uint32_t)y * (uint32_t)g.nx);
+    stencil_columns<T, NS, PF, NT>(u + gg.u_off, un + gg.u_off, ids + gg.id_off, P, S, g, dmask, react, x, y,
+                                   (int)tl.z0, (int)tl.z1);
   }
 }
 
@@ -612,7 +645,8 @@ struct cs_batch {
   uint32_t rfast = 0;
   uint32_t fsm = 0;
   /* launch shape */
-  cs_batch_launch launch{4, 32, 1, 1};
+  cs_batch_launch launch{4, 32, 1, 1, 0};
+  int tile_xcd = 0;                        /* k_batch_tiles' XCD groups       */
   int grid = 0;                            /* persistent blocks               */
   /* captured step periods, keyed by (phase within the period, buffer) */
   bool use_graph = false;
@@ -704,15 +738,26 @@ int upload_tiles(cs_batch* b)
   int grid = prop.multiProcessorCount * std::min(per_cu, 8);
   grid = std::min<int>(grid, (int)((b->ntiles + NQUEUE - 1) / NQUEUE * NQUEUE));
   b->grid = std::max(grid, NQUEUE);
+  /* one block per tile: groups of up to 16 consecutive tiles per XCD,
+   * small enough that every XCD gets at least three groups */
+  b->tile_xcd = 0;
+  for (int G = 16; G >= 2; G /= 2)
+    if ((int64_t)b->ntiles >= 8LL * 3 * G) { b->tile_xcd = G; break; }
   return CS_OK;
 }
 
 template <typename T, int NS, int PF, bool NT, uint32_t FM = BS_DYN>
 void launch_bstencil_k(cs_batch* b, const void* uin, void* uout, uint32_t dmask, uint32_t react)
 {
-  hipLaunchKernelGGL((k_batch_stencil<T, NS, PF, NT, FM>), dim3(b->grid), dim3(BX, b->launch.rows), 0, b->stream,
-                     (const T*)uin, (T*)uout, b->d_ids, (const DevParams<T>*)b->d_params, b->d_geo, b->d_tiles,
-                     b->ntiles, dmask, react, b->d_queue);
+  if (b->launch.persistent)
+    hipLaunchKernelGGL((k_batch_stencil<T, NS, PF, NT, FM>), dim3(b->grid), dim3(BX, b->launch.rows), 0, b->stream,
+                       (const T*)uin, (T*)uout, b->d_ids, (const DevParams<T>*)b->d_params, b->d_geo, b->d_tiles,
+                       b->ntiles, dmask, react, b->d_queue);
+  else
+    hipLaunchKernelGGL((k_batch_tiles<T, NS, PF, NT, FM>), dim3(xcd_launch((int)b->ntiles, b->tile_xcd)),
+                       dim3(BX, b->launch.rows), 0, b->stream, (const T*)uin, (T*)uout, b->d_ids,
+                       (const DevParams<T>*)b->d_params, b->d_geo, b->d_tiles, b->ntiles, dmask, react,
+                       b->tile_xcd);
 }
 
 /* The step masks of the reference's five-slot species layout (Ca, dye,
@@ -1541,6 +1586,7 @@ int cs_batch_set_launch(cs_batch* b, const cs_batch_launch* in)
   if (in->rows < 1 || in->rows > 8) return set_err(CS_EINVAL, "rows must be 1..8");
   if (in->zchunk < 1) return set_err(CS_EINVAL, "zchunk must be >= 1");
   if (in->prefetch < 1 || in->prefetch > 2) return set_err(CS_EINVAL, "prefetch must be 1 or 2");
+  if (in->persistent > 1) return set_err(CS_EINVAL, "persistent must be 0 or 1");
   HIP_TRY(hipSetDevice(b->device));
   HIP_TRY(hipStreamSynchronize(b->stream));
   b->launch = *in;
@@ -1777,7 +1823,8 @@ int cs_batch_autotune(cs_batch* b, uint64_t first, uint32_t reps, cs_batch_launc
   for (uint32_t rows : {1u, 2u, 4u, 8u})
     for (uint32_t zc : {8u, 16u, 32u, (uint32_t)maxz})
       for (uint32_t pf : {1u, 2u})
-        for (uint32_t nt : {0u, 1u}) cand.push_back(cs_batch_launch{rows, zc, pf, nt});
+        for (uint32_t nt : {0u, 1u})
+          for (uint32_t ps : {0u, 1u}) cand.push_back(cs_batch_launch{rows, zc, pf, nt, ps});
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));

@@ -360,11 +360,14 @@ int cs_batch_flux_predictions(cs_batch* b, double* ratio);
 int cs_batch_monitor(cs_batch* b, uint32_t* flags);
 
 /* Launch shape of the ensemble's stencil pass: blocks of 64 x rows
- * threads take tiles of 64*rows consecutive plane columns of one
- * realisation by zchunk planes from eight per-XCD work queues; prefetch
- * and nontemporal as in cs_launch_config.  No reference counterpart. */
+ * threads, each tile 64*rows consecutive plane columns of one realisation
+ * by zchunk planes; prefetch and nontemporal as in cs_launch_config.
+ * persistent = 1 runs one persistent block per resident slot, taking tiles
+ * from eight per-XCD work queues; 0 launches one block per tile, the tiles
+ * numbered in groups per XCD, and leaves the scheduling to the hardware
+ * (same results, bit for bit).  No reference counterpart. */
 typedef struct cs_batch_launch {
-  uint32_t rows, zchunk, prefetch, nontemporal;
+  uint32_t rows, zchunk, prefetch, nontemporal, persistent;
 } cs_batch_launch;
 int cs_batch_get_launch(const cs_batch* b, cs_batch_launch* cfg);
 int cs_batch_set_launch(cs_batch* b, const cs_batch_launch* cfg);

@@ -136,6 +136,35 @@ def test_ensemble_equals_single_contexts_philox(precision):
     ens.close()
 
 
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("shape", [dict(rows=8, zchunk=8), dict(rows=1, zchunk=2)])
+def test_tile_blocks_equal_persistent_blocks(precision, shape):
+    """The stencil pass with one block per tile (persistent = 0, the tiles
+    in groups per XCD once there are enough of them: 64-column tiles two
+    planes deep make 446) and with persistent blocks taking tiles
+    from the queues (persistent = 1) give the same fields and channel
+    trajectories, bit for bit."""
+    ds = _members(fast=True, seed0=61)
+    opts = [cs.Options(seed=500 + k, open={"ryr": [d.open_index]}, precision=precision) for k, d in enumerate(ds)]
+    _say(f"case: tiles vs persistent blocks, {precision}, {shape}")
+    runs = []
+    for persistent in (0, 1):
+        e = cs.Ensemble([(d.geometry, d.model, o) for d, o in zip(ds, opts)], instances=list(range(8)))
+        e.set_launch(prefetch=1, nontemporal=0, persistent=persistent, **shape)
+        runs.append(e)
+    for chunk in (150, 41):
+        for e in runs:
+            e.step(chunk)
+        for i in range(8):
+            for f in range(2):
+                np.testing.assert_array_equal(runs[0].channel_states(i, f), runs[1].channel_states(i, f))
+            for s in range(runs[0].ns):
+                np.testing.assert_array_equal(runs[0].species(i, s), runs[1].species(i, s),
+                                              err_msg=f"member {i} species {s}")
+    for e in runs:
+        e.close()
+
+
 def test_ensemble_graph_replays_equal_direct_launches():
     """Captured periods replayed several times, with eager reductions and
     synchronisations between the calls, equal direct launches."""
