@@ -1,5 +1,5 @@
 #!/usr/bin/env python3
-"""Spark statistics of the closure form, judged by the plain fine-grid method.
+"""Spark statistics of the closure form, checked against the plain fine-grid method.
 
 The same seeded dyads (12 nm geometries from dyad.ensemble_dyad) run three
 ways, each as one ensemble:
