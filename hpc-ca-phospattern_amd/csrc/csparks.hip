@@ -1447,30 +1447,22 @@ int max_fused_zchunk(const Geo& g, size_t esize)
  * chunk and XCD groups below. */
 LaunchCfg default_launch(const Geo& g, size_t esize, int cus)
 {
-  LaunchCfg L{4, 8, 16, 1, 1, 0, 1, 0, 0, 40, 8, 1};
+  LaunchCfg L{4, 8, 16, 1, 1, 0, 1, 0, 0, 32, 8, 1};
+  (void)cus;
   L.zchunk = std::min(L.zchunk, max_zchunk(g, esize));
-  /* k_step2: split z into k equal chunks; with one block per CU at a
-   * time, the pass takes ceil(blocks / CUs) rounds of (chunk + 2) planes
-   * (level 1 recomputes one plane on either side) */
+  /* k_step2: z chunks of 32 planes and XCD groups of up to 8 consecutive
+   * tiles, small enough that every XCD gets at least three groups
+   * (hardware order for small grids).  On the 400x400x200 dyad (FP32,
+   * plain stores, two runs each on one box) a pass takes 459 us at chunk
+   * 32 / groups 8, 463 at 40 / 32, 465 at 40 / 16, 468 at 32 / 16, 469 at
+   * 40 / 8 and 476 at 32 / 32; cs_autotune times the others */
   const int wy = esize == 4 ? 16 : 8;
   const long tiles = (long)((g.nx + BX - 3) / (BX - 2)) * ((g.ny + wy - 3) / (wy - 2));
   const int fzmax = max_fused_zchunk(g, esize);
-  double best = 1e30;
-  for (int k = 1; k <= g.nz; k++) {
-    const int zc = (g.nz + k - 1) / k;
-    if (zc > fzmax) continue;
-    const double rounds = std::ceil((double)(tiles * ((g.nz + zc - 1) / zc)) / cus);
-    const double cost = rounds * (zc + 2);
-    if (cost < best - 1e-9) { best = cost; L.fz = zc; }
-  }
-  L.fz = std::max(1, std::min(L.fz, std::max(fzmax, 1)));
-  /* XCD groups of up to 32 consecutive tiles, small enough that every
-   * XCD gets at least three groups (hardware order for small grids).  On
-   * the 400x400x200 dyad (1015 blocks) groups of 32, 16 and 8 take 501,
-   * 508 and 512 us per pass; 64 takes 533 */
+  L.fz = std::max(1, std::min({L.fz, g.nz, std::max(fzmax, 1)}));
   const long nb = tiles * ((g.nz + L.fz - 1) / L.fz);
   L.fx = 0;
-  for (int G = 32; G >= 2; G /= 2)
+  for (int G = 8; G >= 2; G /= 2)
     if (nb >= 8L * 3 * G) { L.fx = G; break; }
   return L;
 }
@@ -1747,7 +1739,9 @@ void launch_step2_k(cs_context* c, const LaunchCfg& L, const void* uin, void* uo
   const int tz = (g.nz + zc - 1) / zc;
   const int nblocks = tx * ty * tz;
   const F2Voxels fv{c->d_f2_col, c->d_f2_z, c->d_f2_row};
-  hipLaunchKernelGGL((k_step2<T, NS, WY, true, FM1, FM2, FR>), dim3(xcd_launch(nblocks, L.fx)), dim3(BX, WY), 0,
+  /* plain stores: on the 400x400x200 dyad non-temporal ones cost 2% (472 against
+   * 463 us per pass at z chunk 40 and groups of 32) */
+  hipLaunchKernelGGL((k_step2<T, NS, WY, false, FM1, FM2, FR>), dim3(xcd_launch(nblocks, L.fx)), dim3(BX, WY), 0,
                      c->stream, (const T*)uin, (T*)uout, c->d_ids, (const DevParams<T>*)c->d_params, g, fv,
                      (const double*)c->d_f2_fsum, (const uint8_t*)c->d_f2_fany, (T*)c->d_f2_l1, a, b, zc, tx, ty,
                      nblocks, L.fx);
