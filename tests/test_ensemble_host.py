@@ -26,9 +26,10 @@ def test_ensemble_dyads_follow_their_distributions():
         assert len(xy) == n
         # the trigger is a non-phosphorylated RyR nearest the cluster centre
         c = nx // 2
-        dist = [(p[0] - c) ** 2 + (p[1] - c) ** 2 for p in np.vstack([ryr, pryr])]
+        # (pairs are unsigned: signed offsets from the centre)
+        dist = [(int(p[0]) - c) ** 2 + (int(p[1]) - c) ** 2 for p in np.vstack([ryr, pryr])]
         t = ryr[d.open_index]
-        assert (t[0] - c) ** 2 + (t[1] - c) ** 2 == min(dist)
+        assert (int(t[0]) - c) ** 2 + (int(t[1]) - c) ** 2 == min(dist)
     assert 256 - 6 <= min(edges) and max(edges) <= 512 + 6
     assert 9 <= min(counts) and max(counts) <= 100
     assert 0.0 <= min(fracs) and max(fracs) <= 0.5 + 1.0 / 9
