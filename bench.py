@@ -26,9 +26,11 @@ Sub-records of the same line:
   ensemble  512 realisations of 256-512 nm dyads at 6 nm (9-100 RyRs,
             0-50% pRyR) stepped as one ensemble: spark realisations/s over
             their 20 ms runs;
-  closure   realisations/s of 256 dyads of 512-1024 nm (25-150 RyRs) at 12
-            nm with the learned closure gating, against the plain ensemble
-            of the same dyads at 3 nm, over their 50 ms runs.
+  closure   realisations/s of 4096 dyads of 512-1024 nm (25-150 RyRs) at 12
+            nm with the learned closure (gating and flux networks), against
+            the plain ensemble of 16 of the same dyads at 3 nm, over their
+            50 ms runs; then the closure's spark statistics against the 3 nm
+            method's on 1024 small dyads (--ks-count) by two-sample KS.
 Realisations/s = realisations / (steps of one run x measured step time).
 """
 from __future__ import annotations
@@ -175,12 +177,12 @@ def ensemble_record(count=512, steps=100):
                         "achieved": round(gbs, 1), "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4)}}
 
 
-def closure_record(count=4096, fine_count=16, steps=120, fine_steps=40, ks=True):
+def closure_record(count=4096, fine_count=16, steps=120, fine_steps=40, ks=True, ks_count=1024):
     """Config-4 comparison: the 12 nm closure ensemble (both networks)
     against the plain 3 nm ensemble of the same dyads (edges 512-1024 nm,
     25-150 RyRs), step rates at that size; then, with ks, the spark
-    statistics of the closure against the 3 nm method on 256 small dyads
-    (closure_sparks.compare), the KS test the metric is quoted at."""
+    statistics of the closure against the 3 nm method on ks_count small
+    dyads (closure_sparks.compare), the KS test the metric is quoted at."""
     import closure
     t0 = time.perf_counter()
     net, norm, trained = closure.load(CLOSURE_PARAMS)
@@ -224,8 +226,8 @@ def closure_record(count=4096, fine_count=16, steps=120, fine_steps=40, ks=True)
     if ks:
         import closure_sparks
         t1 = time.perf_counter()
-        out = closure_sparks.compare(count=256, edge=(192.0, 256.0), ryr=(9, 30), tstop_ms=0.3, say=log)
-        rec["ks"] = {"workload": "256 dyads of 192-256 nm edges, 9-30 RyRs, fast spark kinetics, 0.3 ms: 12 nm "
+        out = closure_sparks.compare(count=ks_count, edge=(192.0, 256.0), ryr=(9, 30), tstop_ms=0.3, say=log)
+        rec["ks"] = {"workload": f"{ks_count} dyads of 192-256 nm edges, 9-30 RyRs, fast spark kinetics, 0.3 ms: 12 nm "
                                  "closure vs the plain 3 nm method on the same dyads (two-sample KS)",
                      "closure_vs_fine": {k: {"D": round(v["D"], 4), "p": round(v["p"], 4)}
                                          for k, v in out["closure"]["ks_vs_fine"].items()},
@@ -253,6 +255,8 @@ def main():
                     help="fixed launch shape " + ",".join(LAUNCH_KEYS) + " (a prefix of them; skips autotune)")
     ap.add_argument("--no-extra", action="store_true", help="only the headline dyad (no fp64, ensemble, closure)")
     ap.add_argument("--no-ks", action="store_true", help="skip the closure's spark-statistics KS comparison")
+    ap.add_argument("--ks-count", type=int, default=1024,
+                    help="realisations of the KS comparison (each runs at 12 nm and at 3 nm)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB (for scaling traffic counters) and exit")
     args = ap.parse_args()
@@ -356,7 +360,7 @@ def main():
     if not args.no_extra:
         out["fp64"] = fp64_record(args.size, 20)
         out["ensemble"] = ensemble_record()
-        out["closure"] = closure_record(ks=not args.no_ks)
+        out["closure"] = closure_record(ks=not args.no_ks, ks_count=args.ks_count)
     print(json.dumps(out))
 
 
