@@ -63,3 +63,14 @@ def test_ensemble_rejects_mixed_models_without_gpu():
     assert lib.cs_batch_create(arr, 2, C.byref(b)) == -1
     assert b"realisation 1" in lib.cs_last_error() and b"differs" in lib.cs_last_error()
     assert lib.cs_batch_step(None, 1, 1) == -1
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N > 1 ends with status 2 and says
+    why, before it touches a device."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(REPO, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 2 and r.stdout == ""
+    assert "one GPU only" in r.stderr
