@@ -22,7 +22,8 @@ FP64 C oracle on the host cores for a bounded sample of the same workload.
 
 Sub-records of the same line:
   fp64      the same dyad with FP64 fields (one-step passes), the
-            reference's own precision;
+            reference's own precision, and the per-species error of the
+            FP32 fields against the FP64 ones after 200 steps;
   ensemble  512 realisations of 256-512 nm dyads at 6 nm (9-100 RyRs,
             0-50% pRyR) stepped as one ensemble: spark realisations/s over
             their 20 ms runs;
@@ -121,6 +122,39 @@ def fp64_record(size, steps):
             "kernel_ms": round(pass_ms, 5),
             "roofline": {"bound": "hbm", "achieved": round(kern, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                          "frac": round(kern / PEAK_HBM_GBS, 4)}}
+
+
+def fp32_error_record(size, steps=200):
+    """Per-species error of the FP32 fields against FP64 ones on the same
+    dyad after `steps` steps from rest (same seed and channel trajectories;
+    FP32 in its default fused passes, FP64 in one-step passes): the largest
+    |f32 - f64| over the grid divided by the largest deviation of the FP64
+    field from rest.  The FP64 device path equals the FP64 oracle to 1e-10
+    in the parity tests."""
+    d = dyad.cluster_dyad(seed=1, n=tuple(size))
+    sims = {p: cs.Simulation(d.geometry, d.model, cs.Options(h=3.0, seed=1, open={"ryr": [0]}, precision=p))
+            for p in ("f64", "f32")}
+    s64, s32 = sims["f64"], sims["f32"]
+    ns = s64.problem.num_species
+    rest = [s64.species(s) for s in range(ns)]
+    for sim in (s64, s32):
+        sim.step(steps)
+        sim.synchronize()
+    same = all(np.array_equal(s64.channel_states(f), s32.channel_states(f)) for f in range(s64.problem.num_fluxes)
+               if s64.problem.fluxes[f].discrete)
+    errs, devs = [], []
+    for s in range(ns):
+        a, b = s64.species(s), s32.species(s)
+        dev = float(np.max(np.abs(a - rest[s])))
+        err = float(np.max(np.abs(b - a)))
+        devs.append(dev)
+        errs.append(err / dev if dev > 0 else err)
+        del a, b
+    for sim in (s64, s32):
+        sim.close()
+    log("fp32 vs fp64 after %d steps, error / max deviation from rest: %s" % (steps, ", ".join(f"{e:.2e}" for e in errs)))
+    return {"steps": steps, "error_over_max_deviation": [float(f"{e:.3e}") for e in errs],
+            "max_deviation_uM": [float(f"{v:.4g}") for v in devs], "same_channel_trajectories": bool(same)}
 
 
 def ensemble_members(count, h, edge, ryr, seed0, precision, kinetics=None):
@@ -359,6 +393,7 @@ def main():
         out["roofline"]["hbm_frac"] = round(traffic / (kernel_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)
     if not args.no_extra:
         out["fp64"] = fp64_record(args.size, 20)
+        out["fp64"]["fp32_error"] = fp32_error_record(args.size)
         out["ensemble"] = ensemble_record()
         out["closure"] = closure_record(ks=not args.no_ks, ks_count=args.ks_count)
     print(json.dumps(out))
