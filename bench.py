@@ -24,6 +24,9 @@ Sub-records of the same line:
   fp64      the same dyad with FP64 fields (one-step passes), the
             reference's own precision, and the per-species error of the
             FP32 fields against the FP64 ones after 200 steps;
+  g1        the run the reference publishes its own timing for (G1 at 12
+            nm, 1000 steps, 2.6 ms per step on 8 CPU threads), per step on
+            the GPU in FP64 and FP32;
   ensemble  512 realisations of 256-512 nm dyads at 6 nm (9-100 RyRs,
             0-50% pRyR) stepped as one ensemble: spark realisations/s over
             their 20 ms runs;
@@ -155,6 +158,57 @@ def fp32_error_record(size, steps=200):
     log("fp32 vs fp64 after %d steps, error / max deviation from rest: %s" % (steps, ", ".join(f"{e:.2e}" for e in errs)))
     return {"steps": steps, "error_over_max_deviation": [float(f"{e:.3e}") for e in errs],
             "max_deviation_uM": [float(f"{v:.4g}") for v in devs], "same_channel_trajectories": bool(same)}
+
+
+G1_REFERENCE_MS = 2.6   # the reference's main-loop time per step on this run, 8 OpenMP threads
+
+
+def g1_inputs():
+    """The reference's G1 geometry (G1_nonP_0.h5, 84^3 at 12 nm) and its
+    sr1100/ryr55 species file, as the committed fixtures hold them."""
+    golden = os.path.join(REPO, "tests", "golden")
+    meta = json.load(open(os.path.join(golden, "g1_nonP_0.json")))
+    z = np.load(os.path.join(golden, "g1_nonP_0.npz"))
+    g = cs.Geometry(voxels=z["voxels"], h=meta["h"], domain_names=meta["domain_names"],
+                    domain_connections=z["domain_connections"],
+                    boundaries=[cs.Boundary(b["name"], b["type"], z["boundary_" + b["name"]])
+                                for b in meta["boundaries"]],
+                    global_size=tuple(meta["global_size"]))
+    d = json.load(open(os.path.join(golden, "params_sr1100_ryr55.json")))
+    ch = lambda c: cs.ChannelModel(**c) if c else None  # noqa: E731
+    m = cs.SpeciesModel(domain_names=d["domain_names"], species_names=d["species_names"], diffusive=d["diffusive"],
+                        sigma=d["sigma"], init=np.array(d["init"]),
+                        buffers=[[cs.Buffer(**b) for b in dom] for dom in d["buffers"]],
+                        ryr=ch(d["ryr"]), p_ryr=ch(d["p_ryr"]),
+                        serca=cs.SercaModel(**d["serca"]) if d["serca"] else None,
+                        conv=tuple(d["conv"]), fixed=[tuple(x) for x in d["fixed"]])
+    return g, m
+
+
+def g1_record(steps=1000):
+    """The run the reference publishes its own timing for: G1 at 12 nm, 5
+    species slots, dt 1e-4 ms, RyR 24 forced open until 5 ms, 1000 steps;
+    wall time per step on one GPU (gating, stencil, fluxes, SERCA) in FP64
+    and FP32, against the reference's 2.6 ms per step on 8 CPU threads."""
+    g, m = g1_inputs()
+    out = {"workload": "G1 84^3 at 12 nm (592,704 voxels, 31,416 SERCA pairs, 50 RyRs), sr1100/ryr55 species, "
+                       "dt 1e-4 ms, -O ryr 24 -C ryr 5, 1000 steps",
+           "reference_cpu_ms_per_step": G1_REFERENCE_MS, "reference_cpu": "1 MPI rank x 8 OpenMP threads"}
+    for p in ("f64", "f32"):
+        sim = cs.Simulation(g, m, cs.Options(h=12.0, dt=1e-4, seed=1, open={"ryr": [24]}, t_close={"ryr": 5.0},
+                                             precision=p))
+        sim.step(20)
+        sim.synchronize()
+        t1 = time.perf_counter()
+        _, pass_ms, passes, fused = sim.time_passes(steps)
+        sim.synchronize()
+        ms = (time.perf_counter() - t1) * 1e3 / steps
+        sim.close()
+        out[p] = {"ms_per_step": round(ms, 5), "pass_us": round(pass_ms * 1e3, 2), "fused_passes": fused,
+                  "passes": passes, "vs_reference_cpu": round(G1_REFERENCE_MS / ms, 1)}
+        log(f"G1 {p}: {ms * 1e3:.1f} us/step ({fused}/{passes} fused passes, pass {pass_ms * 1e3:.1f} us), "
+            f"{G1_REFERENCE_MS / ms:.0f}x the reference's 2.6 ms/step")
+    return out
 
 
 def ensemble_members(count, h, edge, ryr, seed0, precision, kinetics=None):
@@ -394,6 +448,7 @@ def main():
     if not args.no_extra:
         out["fp64"] = fp64_record(args.size, 20)
         out["fp64"]["fp32_error"] = fp32_error_record(args.size)
+        out["g1"] = g1_record()
         out["ensemble"] = ensemble_record()
         out["closure"] = closure_record(ks=not args.no_ks, ks_count=args.ks_count)
     print(json.dumps(out))

@@ -226,7 +226,9 @@ int cs_fused_available(const cs_context* ctx, char* why, size_t why_len);
  * tile, so the intermediate state stays on chip and the fields cross HBM
  * once per two steps; fuse_zchunk is its planes per block and fuse_xcd
  * its tile order (as xcd_remap).  The other steps, and every step with
- * fuse = 0, run one at a time.  fuse_fixed = 1 (the default) runs the
+ * fuse = 0, run one at a time.  cs_create sets fuse = 1 for FP32 fields
+ * when the problem admits fused passes and their launch has at least
+ * half as many blocks as the device has CUs, else 0.  fuse_fixed = 1 (the default) runs the
  * pairs of step masks of the reference's five-slot species layout (Ca,
  * dye, CMDN-CSQN, ATP, TRPN) in FP32 with kernels compiled for those
  * masks; 0 runs every pass with the masks read at run time (same
