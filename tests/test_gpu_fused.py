@@ -21,7 +21,8 @@ import pytest
 import csparks as cs
 import dyad
 import sparks
-from helpers import DEV_F32, deviation_close, rest_field, species_close, with_serca
+from helpers import (DEV_F32, deviation_close, load_geometry_fixture, load_model_fixture, rest_field, species_close,
+                     with_serca)
 from oracle_py import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -244,3 +245,21 @@ def test_fuse_flag_checked():
     with pytest.raises(RuntimeError):
         sim.set_launch_config(fuse=1, fuse_zchunk=0)
     sim.close()
+
+
+def test_default_form_follows_the_fused_grid_size():
+    """cs_create runs fused passes by default in FP32 only when the fused
+    launch has at least half as many blocks as the device has CUs: the
+    84^3 G1 geometry at 12 nm (36 blocks) takes one-step passes, the
+    200x200x100 dyad (240 blocks) fused ones; FP64 always one-step."""
+    g, m = load_geometry_fixture("g1_nonP_0"), load_model_fixture("params_sr1100_ryr55")
+    d = dyad.cluster_dyad(seed=1, n=(200, 200, 100))
+    cases = [(g, m, 12.0, "f32", 0), (d.geometry, d.model, 3.0, "f32", 1), (d.geometry, d.model, 3.0, "f64", 0)]
+    for geo, model, h, prec, want in cases:
+        sim = cs.Simulation(geo, model, cs.Options(h=h, seed=1, open={"ryr": [0]}, precision=prec))
+        ok, why = sim.fused_available()
+        cfg = sim.launch_config()
+        _say(f"case: {sim.n} {prec}: fused available {ok}, default fuse {cfg['fuse']}")
+        assert ok, why
+        assert cfg["fuse"] == want
+        sim.close()
