@@ -165,24 +165,11 @@ G1_REFERENCE_MS = 2.6   # the reference's main-loop time per step on this run, 8
 
 def g1_inputs():
     """The reference's G1 geometry (G1_nonP_0.h5, 84^3 at 12 nm) and its
-    sr1100/ryr55 species file, as the committed fixtures hold them."""
-    golden = os.path.join(REPO, "tests", "golden")
-    meta = json.load(open(os.path.join(golden, "g1_nonP_0.json")))
-    z = np.load(os.path.join(golden, "g1_nonP_0.npz"))
-    g = cs.Geometry(voxels=z["voxels"], h=meta["h"], domain_names=meta["domain_names"],
-                    domain_connections=z["domain_connections"],
-                    boundaries=[cs.Boundary(b["name"], b["type"], z["boundary_" + b["name"]])
-                                for b in meta["boundaries"]],
-                    global_size=tuple(meta["global_size"]))
-    d = json.load(open(os.path.join(golden, "params_sr1100_ryr55.json")))
-    ch = lambda c: cs.ChannelModel(**c) if c else None  # noqa: E731
-    m = cs.SpeciesModel(domain_names=d["domain_names"], species_names=d["species_names"], diffusive=d["diffusive"],
-                        sigma=d["sigma"], init=np.array(d["init"]),
-                        buffers=[[cs.Buffer(**b) for b in dom] for dom in d["buffers"]],
-                        ryr=ch(d["ryr"]), p_ryr=ch(d["p_ryr"]),
-                        serca=cs.SercaModel(**d["serca"]) if d["serca"] else None,
-                        conv=tuple(d["conv"]), fixed=[tuple(x) for x in d["fixed"]])
-    return g, m
+    sr1100/ryr55 species file, as the committed fixtures hold them (read by
+    the tests' fixture loaders, which use only the package)."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from helpers import load_geometry_fixture, load_model_fixture
+    return load_geometry_fixture("g1_nonP_0"), load_model_fixture("params_sr1100_ryr55")
 
 
 def g1_record(steps=1000):
